@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark: photon packets/s of the persistent HIP transport kernel on one MI355X.
+
+Workload (default): layered skin with eight vessels, 256x256x128 voxels of 0.002 cm, uniform
+illumination, 1e8 packets per step, 256x256 reflectance grid (configs.skin_vessels).  A step is
+one full run of the budget: tally reset, transport kernel, tally read-back to the host.  Inputs
+(voxel grid, optical-property tables, stream seeds and chain schedule) are resident on the device
+before timing starts; seeding is excluded as in the baseline definition.
+
+Prints one JSON line with `roofline` (FP32 VALU work of the transport kernel over the MI355X FP32
+peak, kernel time from HIP events on the engine's stream) and `cpu_baseline` (the CPU oracle on a
+bounded sample of the same workload, all host threads).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# FP32 operations per photon step and per Fresnel event on the reference's common path
+# (src/mc3d_gpu.cu:181-339 and :226-257): add/sub/mul/div/sqrt/rsqrt/log/sin/cos count 1,
+# a*b+c counts 2, compares and selects 0.
+FLOPS_PER_STEP = 108
+FLOPS_PER_FRESNEL = 31
+PEAK_FP32_TFLOPS = 157.3
+
+WORKLOADS = {
+    "skin_vessels": ("skin_vessels", 100_000_000),
+    "skin_slab_sphere": ("skin_slab_sphere", 1_000_000),
+}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="skin_vessels", choices=sorted(WORKLOADS))
+    ap.add_argument("--photons", type=int, default=0, help="override the workload's packet budget")
+    ap.add_argument("--pool-lanes", type=int, default=0)
+    ap.add_argument("--chain-lanes", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the photon budget shards across GPUs with one integer all-reduce of "
+              "the fixed-point tallies, which this build does not run; use --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import ymc3d_amd as y
+    from ymc3d_amd import configs
+
+    fn, default_n = WORKLOADS[args.workload]
+    n_photons = args.photons or default_n
+    scene = getattr(configs, fn)(photons=n_photons)
+    params = dict(scene.params)
+    if args.pool_lanes:
+        params["pool_lanes"] = args.pool_lanes
+    if args.chain_lanes:
+        params["chain_lanes"] = args.chain_lanes
+    tab = y.default_multipliers()
+    eng = y.Engine(0, tab)
+    scene.load(eng)
+
+    # Counter pass (deterministic: the same chains and streams as every timed step).
+    eng.prepare(**params, flags=y.FLAG_COUNTERS)
+    eng.run()
+    counted = eng.read_tally()
+    steps = int(counted.totals[0]["steps"])
+    fresnel = int(counted.totals[0]["fresnel"])
+
+    eng.prepare(**params)
+    for _ in range(args.warmup):
+        eng.run()
+        eng.read_tally()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.run()
+        res = eng.read_tally()
+        kernel_ms.append(res.kernel_ms)
+    elapsed = time.perf_counter() - t0
+    assert int(res.totals[0]["photons"]) == n_photons
+    assert res.totals.tobytes() != b"" and np.array_equal(res.R_w, res.R_w)
+
+    value = args.steps * n_photons / elapsed
+    kms = float(np.mean(kernel_ms))
+    achieved = (steps * FLOPS_PER_STEP + fresnel * FLOPS_PER_FRESNEL) / (kms * 1e-3) / 1e12
+    out = {
+        "metric": "photon packets/s (1 GPU)",
+        "value": value,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": scene.name,
+            "photons_per_step": n_photons,
+            "illumination": "uniform" if params.get("illumination", 0) == 0 else "pencil",
+            "voxels": list(scene.shape),
+            "voxel_cm": scene.voxel[0],
+            "tally": [params.get("tally_nx", 0) or scene.shape[0], params.get("tally_ny", 0) or scene.shape[1]],
+            "pool_lanes": eng.pool_lanes(),
+            "steps_per_photon": steps / n_photons,
+            "fresnel_per_photon": fresnel / n_photons,
+            "kernel_ms": kms,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    eng.close()
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, tab, params, n_photons, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(scene, tab, params, n_photons, seconds):
+    """CPU oracle (all host threads) on a bounded sample of the same workload."""
+    from oracle import oracle as O
+
+    threads = O.lib().ymc_oracle_max_threads()
+    probe = 20_000
+    t0 = time.perf_counter()
+    O.run_scene(scene, tab, num_photons=probe)
+    rate = probe / (time.perf_counter() - t0)
+    sample = int(min(n_photons, max(probe, rate * seconds)))
+    t0 = time.perf_counter()
+    O.run_scene(scene, tab, num_photons=sample)
+    dt = time.perf_counter() - t0
+    return {"value": sample / dt, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} packets of {scene.name} (same geometry, tables, schedule rule), {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

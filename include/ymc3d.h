@@ -1,0 +1,245 @@
+/*
+ * ymc3d.h - C ABI of the MI355X photon-transport engine.
+ *
+ * The engine replaces the reference's host run loop
+ *   void run_3dmc_gpu(geometry_t, opticalprops_t, int num_photons, double *R, double *R_tot,
+ *                     double *T, double *T_tot, float *A, int *num_finished_photons)
+ * (src/mc3d_gpu.h:16, defined src/mc3d_gpu.cu:388-601) together with the two kernels it
+ * launches (photontrack_reinitialize src/mc3d_gpu.cu:74-111, photontrack_step :128-376), the
+ * RNG seeding it calls (rng_init src/mc3d_rng.cu:12-37 -> init_RNG gpumcml_rng/gpumcml_rng.cu:59-118)
+ * and the host tally (photon_detector src/mc3d_photons.cu:151-171).
+ *
+ * Conventions:
+ *  - Plain C types only; every pointer argument is a host pointer unless the name says "dev".
+ *  - Every entry point returns an ymc_status; nothing calls exit() or installs signal handlers.
+ *  - The caller owns every output buffer; the engine owns every device allocation.
+ *  - Weights are tallied as unsigned 64-bit fixed point with scale 2^32 (YMC_FX_ONE), so the
+ *    sums do not depend on the order in which packets finish.
+ *  - One engine is used by one host thread at a time; distinct engines may run concurrently.
+ */
+#ifndef YMC3D_H
+#define YMC3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YMC3D_ABI_VERSION 1
+
+/* Fixed-point scale of every weight sum: value = raw / 2^32. */
+#define YMC_FX_ONE 4294967296.0
+
+/* Row 0 of every optical-property table is the ambient medium (src/mc3d_io.cu:38-42). */
+#define YMC_TISSUE_AIR 0
+
+/* Physics constants of the reference (src/mc3d_types.h:11-13). */
+#define YMC_SURVIVAL_CHANCE 0.1
+#define YMC_W_THRESHOLD 0.0001
+#define YMC_UZ_THRESH 0.99999
+
+typedef enum ymc_status {
+  YMC_OK = 0,
+  YMC_ERR_ARG = 1,       /* null pointer, bad size or out-of-range parameter */
+  YMC_ERR_IO = 2,        /* file missing, unreadable or short */
+  YMC_ERR_FORMAT = 3,    /* header or content violates the file format */
+  YMC_ERR_SEED = 4,      /* seed rejected by the init_RNG validity rule (gpumcml_rng.cu:88) */
+  YMC_ERR_TABLE = 5,     /* multiplier table missing, too short or invalid */
+  YMC_ERR_DEVICE = 6,    /* HIP runtime error (message via ymc_last_error) */
+  YMC_ERR_NOMEM = 7,     /* host or device allocation failed */
+  YMC_ERR_CANCELLED = 8, /* run stopped by the cancel flag; tallies hold the finished packets */
+  YMC_ERR_STATE = 9      /* call order violated (e.g. run before geometry was set) */
+} ymc_status;
+
+enum { YMC_ILLUM_UNIFORM = 0, YMC_ILLUM_PENCIL = 1 };
+enum { YMC_FRESNEL_YMC3D = 0, YMC_FRESNEL_EXACT = 1 };
+
+/* Run flags. */
+#define YMC_FLAG_COUNTERS 0x1u /* collect steps / Fresnel / absorption / roulette counters */
+#define YMC_FLAG_RECORD 0x2u   /* write one ymc_photon_record per packet (test hook) */
+
+/* Voxel geometry.  Mirrors geometry_t (src/mc3d_types.h:38-43) after geometry_read_from_file
+ * (src/mc3d_io.cu:107-169) has remapped the file values: file value v in 0..254 becomes row
+ * v+1, file value 255 becomes row 0 (ambient).  Rows therefore fit in uint8. */
+typedef struct ymc_geometry {
+  int32_t nx, ny, nz;     /* voxel counts */
+  float dx, dy, dz;       /* voxel size, same length unit as 1/mu */
+  const uint8_t *tissue;  /* nz*ny*nx rows, x fastest */
+} ymc_geometry;
+
+/* Optical properties.  Mirrors opticalprops_t (src/mc3d_types.h:28-36): num_types rows
+ * including row 0 = ambient (mua=mus=g=0, n=1).  num_tables > 1 stacks independent
+ * tables (wavelengths / property sets) for a batched run. */
+typedef struct ymc_optprops {
+  int32_t num_types;   /* rows per table, including row 0 */
+  int32_t num_tables;  /* >= 1 */
+  const float *mua;    /* [num_tables * num_types] absorption coefficient */
+  const float *mus;    /* scattering coefficient */
+  const float *g;      /* Henyey-Greenstein anisotropy */
+  const float *n;      /* refractive index */
+} ymc_optprops;
+
+/* Run parameters.  Zero-initialise and set what you need; zeros select the defaults. */
+typedef struct ymc_params {
+  uint64_t num_photons;        /* packets per table (used when budgets == NULL) */
+  const uint64_t *budgets;     /* optional [num_tables] packet budgets */
+  uint64_t seed;               /* init_RNG seed; 0 selects 1 */
+  const volatile int32_t *cancel; /* optional host flag polled by the device; nonzero stops launches */
+  int32_t illumination;        /* YMC_ILLUM_UNIFORM (src/mc3d_gpu.cu:88-89) or _PENCIL (:92-94) */
+  float pencil_x, pencil_y;    /* beam position; negative selects the grid centre */
+  int32_t tally_nx, tally_ny;  /* reflectance/transmittance grid; 0 selects geometry nx, ny */
+  float tally_dx, tally_dy;    /* bin size; 0 selects geometry dx, dy */
+  int32_t radial_bins;         /* radial reflectance bins; 0 disables the radial tally */
+  float radial_dr;             /* radial bin width */
+  float radial_x0, radial_y0;  /* radial centre; negative selects the beam position */
+  int32_t fresnel_mode;        /* YMC_FRESNEL_YMC3D (reference formula) or _EXACT */
+  int32_t kill_zero_step;      /* zero-step kill of src/mc3d_gpu.cu:343: 1 = at the end of each 45-step
+                                  launch as the reference (default), 2 = every step, 0 = never */
+  int32_t chain_lanes;         /* nominal lane count of the chain schedule; 0 selects 524288 */
+  int32_t pool_lanes;          /* resident lanes (device); 0 selects full occupancy */
+  int32_t compact_interval;    /* steps between pool compaction points; 0 selects default */
+  uint32_t flags;              /* YMC_FLAG_* */
+} ymc_params;
+
+/* Per-table totals (fixed point, scale YMC_FX_ONE). */
+typedef struct ymc_totals {
+  uint64_t photons;     /* packets finished */
+  uint64_t R_w, R_w2;   /* reflected weight (z <= 0 at exit), anywhere, and its square */
+  uint64_t T_w, T_w2;   /* weight that finished with z > 0 */
+  uint64_t R_n, T_n;    /* packets with positive weight in R / T */
+  uint64_t A_w;         /* absorbed weight (implicit capture, src/mc3d_gpu.cu:262-263) */
+  int64_t roulette_w;   /* net weight added by Russian roulette (src/mc3d_gpu.cu:275-277) */
+  uint64_t killed_w;    /* weight removed by the non-finite / zero-step kill (:343-353) */
+  uint64_t killed;      /* packets killed that way */
+  uint64_t steps;       /* photon steps (iterations of src/mc3d_gpu.cu:175-340) */
+  uint64_t fresnel;     /* Fresnel events (src/mc3d_gpu.cu:226-257) */
+} ymc_totals;
+
+/* Tally destination; any pointer may be NULL.  Sizes: maps num_tables*tally_ny*tally_nx
+ * (x fastest), radial num_tables*radial_bins, totals num_tables. */
+typedef struct ymc_tally {
+  uint64_t *R_w, *R_w2;   /* per-bin reflectance weight sums (src/mc3d_photons.cu:160-161) */
+  uint32_t *R_n;
+  uint64_t *T_w, *T_w2;   /* per-bin transmittance (src/mc3d_photons.cu:166-167) */
+  uint32_t *T_n;
+  uint64_t *Rr_w, *Rr_w2; /* radial reflectance */
+  uint32_t *Rr_n;
+  ymc_totals *totals;
+} ymc_tally;
+
+/* Final state of one packet (YMC_FLAG_RECORD). */
+typedef struct ymc_photon_record {
+  float x, y, z, w;  /* exit position and weight */
+  uint32_t steps;    /* steps taken by this packet */
+  int32_t fate;      /* 0 reflected (z<=0), 1 finished with z>0, 2 killed */
+} ymc_photon_record;
+
+/* One chain: `count` consecutive packets of table `table`, all drawn from one RNG stream. */
+typedef struct ymc_chain {
+  uint32_t count;
+  uint32_t table;
+} ymc_chain;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Host utilities (no device needed).                                                         */
+
+const char *ymc_last_error(void);
+int32_t ymc_abi_version(void);
+
+/* Multiplier table.  Entry 0 is the seeding multiplier `begin`, entries 1.. are the stream
+ * multipliers.  Replaces the file read of init_RNG (gpumcml_rng/gpumcml_rng.cu:72-98).
+ * ymc_multipliers_generate writes the `count` largest a < 2^32 with a*2^32-1 and a*2^31-1
+ * both prime, in descending order. */
+ymc_status ymc_multipliers_generate(uint32_t count, uint32_t *out);
+/* Reads the first column of a safeprimes_base32.txt-style text file (up to cap lines). */
+ymc_status ymc_multipliers_read_text(const char *path, uint32_t *out, uint32_t cap, uint32_t *count);
+
+/* Stream seeding: init_RNG (gpumcml_rng/gpumcml_rng.cu:59-118) for n streams; stream i uses
+ * multiplier table[1 + i % (table_len-1)].  Returns YMC_ERR_SEED for an invalid seed. */
+ymc_status ymc_seed_streams(const uint32_t *table, uint32_t table_len, uint64_t seed,
+                            uint64_t n, uint64_t *x_out, uint32_t *a_out);
+
+/* Chain schedule: splits the budgets into chains in dispatch order (guided: chain length
+ * ceil(remaining / (2*chain_lanes)), tables interleaved).  Call with out == NULL to get the
+ * count.  The schedule depends only on the budgets and chain_lanes. */
+ymc_status ymc_build_chains(const uint64_t *budgets, int32_t num_tables, int32_t chain_lanes,
+                            ymc_chain *out, uint64_t cap, uint64_t *count);
+
+/* Binary formats (src/mc3d_io.cu).  Geometry/optprops readers allocate with malloc; release
+ * with ymc_free.  ymc_read_geometry applies the +1 / 255->0 remap of src/mc3d_io.cu:156-163. */
+ymc_status ymc_read_geometry(const char *path, int32_t *nx, int32_t *ny, int32_t *nz, double *dx,
+                             double *dy, double *dz, uint8_t **tissue);
+ymc_status ymc_write_geometry(const char *path, int32_t nx, int32_t ny, int32_t nz, double dx,
+                              double dy, double dz, const int32_t *file_values);
+/* Reads T records of (mua, mus, g, n) doubles and prepends the ambient row
+ * (src/mc3d_io.cu:16-66); outputs have T+1 entries. */
+ymc_status ymc_read_optprops(const char *path, int32_t *num_types, float **mua, float **mus,
+                             float **g, float **n);
+ymc_status ymc_write_optprops(const char *path, int32_t num_records, const double *records);
+/* save_property layout (src/mc3d_io.cu:175-216): int32 dims, int32 nx, ny[, nz], f64 dx, dy[, dz],
+ * f64 data. */
+ymc_status ymc_write_property(const char *path, int32_t dims, int32_t nx, int32_t ny, int32_t nz,
+                              double dx, double dy, double dz, const double *data);
+void ymc_free(void *p);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Device engine.                                                                             */
+
+typedef struct ymc_engine ymc_engine;
+
+/* Creates an engine on HIP device `device`.  The multiplier table is copied; table_len >= 2. */
+ymc_status ymc_engine_create(int32_t device, const uint32_t *table, uint32_t table_len,
+                             ymc_engine **out);
+void ymc_engine_destroy(ymc_engine *e);
+
+/* Uploads the voxel grid (the reference's 3-D texture, src/mc3d_gpu.cu:398-421). */
+ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g);
+/* Uploads the optical-property tables (opticalprops_transfer_to_device, src/mc3d_io.cu:70-86). */
+ymc_status ymc_engine_set_optprops(ymc_engine *e, const ymc_optprops *p);
+
+/* Seeds the streams and uploads the chain schedule for `params` (host work, not timed). */
+ymc_status ymc_engine_prepare(ymc_engine *e, const ymc_params *params);
+/* Same, with an explicit chain list and stream states (test hook / external schedulers). */
+ymc_status ymc_engine_prepare_chains(ymc_engine *e, const ymc_params *params,
+                                     const ymc_chain *chains, const uint64_t *x, const uint32_t *a,
+                                     uint64_t n_chains);
+/* Runs the prepared photon budget: clears the device tallies, launches the persistent
+ * transport kernel and waits for it.  Tallies stay on the device. */
+ymc_status ymc_engine_run(ymc_engine *e);
+/* Copies the device tallies into `out`. */
+ymc_status ymc_engine_read_tally(ymc_engine *e, ymc_tally *out);
+/* Copies per-packet records of the last run made with YMC_FLAG_RECORD. */
+ymc_status ymc_engine_read_records(ymc_engine *e, ymc_photon_record *out, uint64_t cap);
+/* Milliseconds of the last ymc_engine_run transport kernel, from HIP events on its stream. */
+ymc_status ymc_engine_last_kernel_ms(ymc_engine *e, double *ms);
+/* Resident lanes used by the last run. */
+ymc_status ymc_engine_pool_lanes(ymc_engine *e, int32_t *lanes);
+
+/* RNG test hook: advances n_streams MWC streams by n_draws draws on the device and returns
+ * every draw as the float of rand_MWC_co (kind 0, gpumcml_rng.cu:38-46) or rand_MWC_oc
+ * (kind 1, :51-54) in out[draw * n_streams + stream], plus the final states. */
+ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t *a,
+                               uint32_t n_streams, uint32_t n_draws, int32_t kind, float *out,
+                               uint64_t *x_final);
+
+/* One-shot drop-in for run_3dmc_gpu (src/mc3d_gpu.cu:388): same arguments as plain pointers.
+ * tissue holds remapped int32 rows (as geometry_t.tissue_type); optprops arrays have
+ * num_types entries with row 0 = ambient.  R and T receive raw weight sums per geometry
+ * bin (accumulated, as the reference does), R_tot / T_tot the sums over all exits, A is
+ * unused (the reference never fills it, src/mc3d_gpu.cu:265), num_finished the packet
+ * count.  Uniform illumination and the reference constants; seed and multiplier table
+ * as given. */
+ymc_status ymc_run_3dmc_gpu(int32_t nx, int32_t ny, int32_t nz, float dx, float dy, float dz,
+                            const int32_t *tissue, int32_t num_types, const float *mua,
+                            const float *mus, const float *g, const float *n, int32_t num_photons,
+                            uint64_t seed, const uint32_t *table, uint32_t table_len, double *R,
+                            double *R_tot, double *T, double *T_tot, float *A,
+                            int32_t *num_finished);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YMC3D_H */

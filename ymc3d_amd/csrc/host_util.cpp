@@ -1,0 +1,396 @@
+// host_util.cpp - host side of the engine: multiplier table, stream seeding, chain schedule,
+// binary file formats and error reporting.  No device code.
+#include "../../include/ymc3d.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "host_util.h"
+
+namespace ymc {
+
+static thread_local char g_last_error[512];
+
+ymc_status fail(ymc_status st, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+const char *last_error() { return g_last_error; }
+
+// ------------------------------------------------------------------------------------------
+// Multiplier generation.  A multiply-with-carry generator x' = a*x + c (base 2^32) has
+// maximal period (a*2^32-2)/2 when a*2^32-1 and a*2^31-1 are both prime; this is the property
+// the entries of GPU-MCML's safeprimes_base32.txt satisfy.  Candidates are scanned downward
+// from 2^32-1 in segments: a sieve removes a with a small prime factor in either number, then
+// a deterministic Miller-Rabin test (bases valid for every n < 2^64) decides the survivors.
+
+namespace {
+
+inline uint64_t mul_mod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)((unsigned __int128)a * b % m); }
+
+bool miller_rabin(uint64_t n) {
+  if (n < 4) return n == 2 || n == 3;
+  if ((n & 1) == 0) return false;
+  uint64_t d = n - 1;
+  int s = 0;
+  while ((d & 1) == 0) {
+    d >>= 1;
+    ++s;
+  }
+  static const uint64_t bases[] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+  for (uint64_t b : bases) {
+    uint64_t a = b % n;
+    if (a == 0) continue;
+    uint64_t x = 1, p = a, e = d;
+    while (e) {
+      if (e & 1) x = mul_mod(x, p, n);
+      p = mul_mod(p, p, n);
+      e >>= 1;
+    }
+    if (x == 1 || x == n - 1) continue;
+    bool witness = true;
+    for (int r = 1; r < s; ++r) {
+      x = mul_mod(x, x, n);
+      if (x == n - 1) {
+        witness = false;
+        break;
+      }
+    }
+    if (witness) return false;
+  }
+  return true;
+}
+
+struct SievePrime {
+  uint32_t p, r32, r31;  // a = r32 (mod p) <=> p | a*2^32-1 ; a = r31 (mod p) <=> p | a*2^31-1
+};
+
+std::vector<SievePrime> sieve_primes(uint32_t limit) {
+  std::vector<char> comp(limit + 1, 0);
+  std::vector<SievePrime> out;
+  for (uint32_t p = 3; p <= limit; p += 2) {
+    if (comp[p]) continue;
+    for (uint64_t q = (uint64_t)p * p; q <= limit; q += 2 * p) comp[q] = 1;
+    // inverse of 2 mod p is (p+1)/2; inverse of 2^k is its k-th power
+    uint64_t inv2 = (p + 1) / 2, inv31 = 1;
+    for (int k = 0; k < 31; ++k) inv31 = inv31 * inv2 % p;
+    uint64_t inv32 = inv31 * inv2 % p;
+    out.push_back({p, (uint32_t)inv32, (uint32_t)inv31});
+  }
+  return out;
+}
+
+// Valid multipliers in (hi - len, hi], descending.
+void scan_segment(uint32_t hi, uint32_t len, const std::vector<SievePrime> &primes, std::vector<uint32_t> &out) {
+  uint32_t lo = hi - len + 1;  // inclusive
+  std::vector<char> bad(len, 0);
+  for (const SievePrime &sp : primes) {
+    for (uint32_t r : {sp.r32, sp.r31}) {
+      // first a >= lo with a = r (mod p)
+      uint32_t m = lo % sp.p;
+      uint32_t off = (r + sp.p - m) % sp.p;
+      for (uint64_t idx = off; idx < len; idx += sp.p) bad[idx] = 1;
+    }
+  }
+  for (int64_t idx = (int64_t)len - 1; idx >= 0; --idx) {
+    if (bad[idx]) continue;
+    uint32_t a = lo + (uint32_t)idx;
+    uint64_t n31 = ((uint64_t)a << 31) - 1;
+    if (!miller_rabin(n31)) continue;
+    uint64_t n32 = ((uint64_t)a << 32) - 1;
+    if (!miller_rabin(n32)) continue;
+    out.push_back(a);
+  }
+}
+
+}  // namespace
+
+ymc_status generate_multipliers(uint32_t count, uint32_t *out) {
+  if (!out || count == 0) return fail(YMC_ERR_ARG, "generate_multipliers: bad arguments");
+  const uint32_t kSeg = 1u << 20;
+  static const std::vector<SievePrime> primes = sieve_primes(1u << 15);
+  unsigned nthreads = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  uint32_t have = 0;
+  uint64_t hi = 0xffffffffull;
+  while (have < count) {
+    // one batch of nthreads consecutive segments, scanned in parallel, merged in order
+    std::vector<std::vector<uint32_t>> res(nthreads);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nthreads; ++t) {
+      uint64_t seg_hi = hi - (uint64_t)t * kSeg;
+      if (seg_hi < (1ull << 31) + kSeg) break;
+      pool.emplace_back([&, t, seg_hi] { scan_segment((uint32_t)seg_hi, kSeg, primes, res[t]); });
+    }
+    if (pool.empty()) return fail(YMC_ERR_TABLE, "multiplier search ran below 2^31");
+    for (auto &th : pool) th.join();
+    for (size_t t = 0; t < pool.size() && have < count; ++t)
+      for (uint32_t a : res[t]) {
+        if (have == count) break;
+        out[have++] = a;
+      }
+    hi -= (uint64_t)pool.size() * kSeg;
+  }
+  return YMC_OK;
+}
+
+ymc_status read_multipliers_text(const char *path, uint32_t *out, uint32_t cap, uint32_t *count) {
+  if (!path || !out || !count) return fail(YMC_ERR_ARG, "read_multipliers_text: null argument");
+  FILE *f = std::fopen(path, "r");
+  if (!f) return fail(YMC_ERR_IO, "cannot open multiplier file %s", path);
+  uint32_t n = 0;
+  unsigned long long a, b, c;
+  while (n < cap && std::fscanf(f, "%llu %llu %llu", &a, &b, &c) == 3) {
+    if (a < 2 || a > 0xffffffffull) {
+      std::fclose(f);
+      return fail(YMC_ERR_TABLE, "multiplier file %s: bad value on line %u", path, n + 1);
+    }
+    out[n++] = (uint32_t)a;
+  }
+  std::fclose(f);
+  *count = n;
+  if (n < 2) return fail(YMC_ERR_TABLE, "multiplier file %s has fewer than 2 lines", path);
+  return YMC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Stream seeding, gpumcml_rng/gpumcml_rng.cu:59-118.  The first table entry (`begin`) drives a
+// seeding MWC stream; each output stream i takes multiplier table[1 + i mod (len-1)] and draws
+// its carry c = floor(u/2^32 * a) and value x from two consecutive seeding outputs, rejecting
+// the forbidden states (x == 0, c >= a-1, low word 0xffffffff) exactly as the reference loop.
+
+ymc_status seed_streams(const uint32_t *table, uint32_t table_len, uint64_t seed, uint64_t n, uint64_t *x_out,
+                        uint32_t *a_out) {
+  if (!table || table_len < 2 || (n && (!x_out || !a_out)))
+    return fail(YMC_ERR_ARG, "seed_streams: need a table of >= 2 multipliers and output arrays");
+  const uint32_t begin = table[0];
+  const uint64_t nmult = table_len - 1;
+  uint64_t s = seed;
+  const uint32_t s_hi = (uint32_t)(s >> 32), s_lo = (uint32_t)s;
+  if (s == 0 || s_hi >= begin - 1 || s_lo >= 0xffffffffu)
+    return fail(YMC_ERR_SEED, "seed %llu is not a valid MWC state for multiplier %u", (unsigned long long)seed, begin);
+  auto next = [&]() -> uint32_t {
+    s = (uint64_t)(uint32_t)s * begin + (s >> 32);
+    return (uint32_t)s;
+  };
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t a = table[1 + i % nmult];
+    uint64_t st;
+    for (;;) {
+      uint32_t u = next();
+      uint64_t carry = (uint32_t)std::floor(((double)u / 4294967296.0) * (double)a);
+      uint32_t v = next();
+      st = (carry << 32) + v;
+      bool bad = st == 0 || (uint32_t)(st >> 32) >= a - 1 || (uint32_t)st == 0xffffffffu;
+      if (!bad) break;
+    }
+    x_out[i] = st;
+    a_out[i] = a;
+  }
+  return YMC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Chain schedule.  Chains are handed to lanes in order; their lengths shrink with the work
+// left (guided scheduling) so the last chains are single packets and the end of the run is
+// not held up by a long chain.  The schedule depends only on the budgets and chain_lanes, never
+// on the device, which keeps results identical for any pool size.
+
+ymc_status build_chains(const uint64_t *budgets, int32_t num_tables, int32_t chain_lanes, ymc_chain *out,
+                        uint64_t cap, uint64_t *count) {
+  if (!budgets || num_tables < 1 || chain_lanes < 1 || !count) return fail(YMC_ERR_ARG, "build_chains: bad arguments");
+  std::vector<uint64_t> left(budgets, budgets + num_tables);
+  uint64_t total = 0;
+  for (uint64_t b : left) total += b;
+  const uint64_t denom = 2ull * (uint64_t)chain_lanes;
+  uint64_t n = 0;
+  int w = 0;
+  while (total) {
+    while (left[w] == 0) w = (w + 1) % num_tables;
+    uint64_t len = std::min<uint64_t>((total + denom - 1) / denom, 0xffffffffull);
+    len = std::min(len, left[w]);
+    if (out) {
+      if (n >= cap) return fail(YMC_ERR_ARG, "build_chains: output capacity %llu too small", (unsigned long long)cap);
+      out[n] = ymc_chain{(uint32_t)len, (uint32_t)w};
+    }
+    ++n;
+    left[w] -= len;
+    total -= len;
+    w = (w + 1) % num_tables;
+  }
+  *count = n;
+  return YMC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Binary formats, src/mc3d_io.cu.
+
+namespace {
+struct File {
+  FILE *f;
+  explicit File(FILE *p) : f(p) {}
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
+
+ymc_status read_geometry(const char *path, int32_t *nx, int32_t *ny, int32_t *nz, double *dx, double *dy,
+                         double *dz, uint8_t **tissue) {
+  if (!path || !nx || !ny || !nz || !dx || !dy || !dz || !tissue) return fail(YMC_ERR_ARG, "read_geometry: null argument");
+  File F(std::fopen(path, "rb"));
+  if (!F.f) return fail(YMC_ERR_IO, "cannot open geometry file %s", path);
+  int32_t hdr[4];
+  double d[3];
+  if (std::fread(hdr, 4, 4, F.f) != 4 || std::fread(d, 8, 3, F.f) != 3)
+    return fail(YMC_ERR_IO, "geometry file %s: short header", path);
+  if (hdr[0] != 3) return fail(YMC_ERR_FORMAT, "geometry file %s: dimension %d is not 3", path, hdr[0]);
+  if (hdr[1] <= 0 || hdr[2] <= 0 || hdr[3] <= 0) return fail(YMC_ERR_FORMAT, "geometry file %s: bad size", path);
+  if (!(d[0] > 0) || !(d[1] > 0) || !(d[2] > 0)) return fail(YMC_ERR_FORMAT, "geometry file %s: bad voxel size", path);
+  size_t nv = (size_t)hdr[1] * hdr[2] * hdr[3];
+  std::vector<int32_t> raw(nv);
+  if (std::fread(raw.data(), 4, nv, F.f) != nv) return fail(YMC_ERR_IO, "geometry file %s: short voxel data", path);
+  uint8_t *t = (uint8_t *)std::malloc(nv);
+  if (!t) return fail(YMC_ERR_NOMEM, "read_geometry: out of memory");
+  for (size_t v = 0; v < nv; ++v) {
+    int32_t val = raw[v];
+    if (val < 0 || val > 255) {
+      std::free(t);
+      return fail(YMC_ERR_FORMAT, "geometry file %s: voxel value %d outside 0..255", path, val);
+    }
+    t[v] = (val == 255) ? (uint8_t)YMC_TISSUE_AIR : (uint8_t)(val + 1);  // src/mc3d_io.cu:156-163
+  }
+  *nx = hdr[1];
+  *ny = hdr[2];
+  *nz = hdr[3];
+  *dx = d[0];
+  *dy = d[1];
+  *dz = d[2];
+  *tissue = t;
+  return YMC_OK;
+}
+
+ymc_status write_geometry(const char *path, int32_t nx, int32_t ny, int32_t nz, double dx, double dy, double dz,
+                          const int32_t *vals) {
+  if (!path || !vals || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "write_geometry: bad arguments");
+  File F(std::fopen(path, "wb"));
+  if (!F.f) return fail(YMC_ERR_IO, "cannot create %s", path);
+  int32_t hdr[4] = {3, nx, ny, nz};
+  double d[3] = {dx, dy, dz};
+  size_t nv = (size_t)nx * ny * nz;
+  if (std::fwrite(hdr, 4, 4, F.f) != 4 || std::fwrite(d, 8, 3, F.f) != 3 || std::fwrite(vals, 4, nv, F.f) != nv)
+    return fail(YMC_ERR_IO, "short write to %s", path);
+  return YMC_OK;
+}
+
+ymc_status read_optprops(const char *path, int32_t *num_types, float **mua, float **mus, float **g, float **n) {
+  if (!path || !num_types || !mua || !mus || !g || !n) return fail(YMC_ERR_ARG, "read_optprops: null argument");
+  File F(std::fopen(path, "rb"));
+  if (!F.f) return fail(YMC_ERR_IO, "cannot open optical-properties file %s", path);
+  int32_t T;
+  if (std::fread(&T, 4, 1, F.f) != 1) return fail(YMC_ERR_IO, "optical-properties file %s: short header", path);
+  if (T < 1 || T > 255) return fail(YMC_ERR_FORMAT, "optical-properties file %s: %d tissue types", path, T);
+  std::vector<double> rec((size_t)T * 4);
+  if (std::fread(rec.data(), 8, rec.size(), F.f) != rec.size())
+    return fail(YMC_ERR_IO, "optical-properties file %s: short data", path);
+  const int32_t rows = T + 1;
+  float *arr[4];
+  for (int q = 0; q < 4; ++q) {
+    arr[q] = (float *)std::malloc(sizeof(float) * rows);
+    if (!arr[q]) {
+      for (int r = 0; r < q; ++r) std::free(arr[r]);
+      return fail(YMC_ERR_NOMEM, "read_optprops: out of memory");
+    }
+  }
+  // row 0: ambient medium (src/mc3d_io.cu:38-42)
+  arr[0][0] = 0;
+  arr[1][0] = 0;
+  arr[2][0] = 0;
+  arr[3][0] = 1.0f;
+  for (int k = 1; k < rows; ++k)  // records in the order mua, mus, g, n (src/mc3d_io.cu:45-52)
+    for (int q = 0; q < 4; ++q) arr[q][k] = (float)rec[(size_t)(k - 1) * 4 + q];
+  *num_types = rows;
+  *mua = arr[0];
+  *mus = arr[1];
+  *g = arr[2];
+  *n = arr[3];
+  return YMC_OK;
+}
+
+ymc_status write_optprops(const char *path, int32_t num_records, const double *records) {
+  if (!path || !records || num_records < 1) return fail(YMC_ERR_ARG, "write_optprops: bad arguments");
+  File F(std::fopen(path, "wb"));
+  if (!F.f) return fail(YMC_ERR_IO, "cannot create %s", path);
+  size_t nd = (size_t)num_records * 4;
+  if (std::fwrite(&num_records, 4, 1, F.f) != 1 || std::fwrite(records, 8, nd, F.f) != nd)
+    return fail(YMC_ERR_IO, "short write to %s", path);
+  return YMC_OK;
+}
+
+ymc_status write_property(const char *path, int32_t dims, int32_t nx, int32_t ny, int32_t nz, double dx, double dy,
+                          double dz, const double *data) {
+  if (!path || !data || (dims != 2 && dims != 3) || nx <= 0 || ny <= 0 || (dims == 3 && nz <= 0))
+    return fail(YMC_ERR_ARG, "write_property: bad arguments");
+  File F(std::fopen(path, "wb"));
+  if (!F.f) return fail(YMC_ERR_IO, "cannot create %s", path);
+  int32_t ih[4] = {dims, nx, ny, nz};
+  double dh[3] = {dx, dy, dz};
+  size_t cnt = (size_t)nx * ny * (dims == 3 ? nz : 1);
+  if (std::fwrite(ih, 4, 1 + dims, F.f) != (size_t)(1 + dims) || std::fwrite(dh, 8, dims, F.f) != (size_t)dims ||
+      std::fwrite(data, 8, cnt, F.f) != cnt)
+    return fail(YMC_ERR_IO, "short write to %s", path);
+  return YMC_OK;
+}
+
+}  // namespace ymc
+
+// ============================================================================================
+// C ABI wrappers for the host utilities.
+
+extern "C" {
+
+const char *ymc_last_error(void) { return ymc::last_error(); }
+int32_t ymc_abi_version(void) { return YMC3D_ABI_VERSION; }
+
+ymc_status ymc_multipliers_generate(uint32_t count, uint32_t *out) { return ymc::generate_multipliers(count, out); }
+ymc_status ymc_multipliers_read_text(const char *path, uint32_t *out, uint32_t cap, uint32_t *count) {
+  return ymc::read_multipliers_text(path, out, cap, count);
+}
+ymc_status ymc_seed_streams(const uint32_t *table, uint32_t table_len, uint64_t seed, uint64_t n, uint64_t *x_out,
+                            uint32_t *a_out) {
+  return ymc::seed_streams(table, table_len, seed, n, x_out, a_out);
+}
+ymc_status ymc_build_chains(const uint64_t *budgets, int32_t num_tables, int32_t chain_lanes, ymc_chain *out,
+                            uint64_t cap, uint64_t *count) {
+  return ymc::build_chains(budgets, num_tables, chain_lanes, out, cap, count);
+}
+ymc_status ymc_read_geometry(const char *path, int32_t *nx, int32_t *ny, int32_t *nz, double *dx, double *dy,
+                             double *dz, uint8_t **tissue) {
+  return ymc::read_geometry(path, nx, ny, nz, dx, dy, dz, tissue);
+}
+ymc_status ymc_write_geometry(const char *path, int32_t nx, int32_t ny, int32_t nz, double dx, double dy, double dz,
+                              const int32_t *file_values) {
+  return ymc::write_geometry(path, nx, ny, nz, dx, dy, dz, file_values);
+}
+ymc_status ymc_read_optprops(const char *path, int32_t *num_types, float **mua, float **mus, float **g, float **n) {
+  return ymc::read_optprops(path, num_types, mua, mus, g, n);
+}
+ymc_status ymc_write_optprops(const char *path, int32_t num_records, const double *records) {
+  return ymc::write_optprops(path, num_records, records);
+}
+ymc_status ymc_write_property(const char *path, int32_t dims, int32_t nx, int32_t ny, int32_t nz, double dx,
+                              double dy, double dz, const double *data) {
+  return ymc::write_property(path, dims, nx, ny, nz, dx, dy, dz, data);
+}
+void ymc_free(void *p) { std::free(p); }
+
+}  // extern "C"

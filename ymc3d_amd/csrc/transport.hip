@@ -1,0 +1,1058 @@
+// transport.hip - persistent photon-transport kernel for gfx950 and the device engine.
+//
+// One lane carries one photon packet at a time.  Lanes take chains of packets from a global
+// counter (a chain owns one multiply-with-carry stream, so a packet's random numbers do not
+// depend on which lane runs it or when), run each packet step by step until it freezes, tally
+// it with 64-bit fixed-point atomics and start the next packet of the chain.  The kernel stays
+// resident until every chain is done.
+//
+// Per step the lane follows photontrack_step (src/mc3d_gpu.cu:181-339) with the reference's
+// draw order (optional Fresnel draw, roulette, HG cos, HG phi, new step length), so a packet's
+// trajectory matches the CPU oracle draw for draw.  What differs is how the arithmetic is
+// issued on CDNA4:
+//  - optical properties of every (table, tissue) row sit in LDS with derived constants
+//    (mu_t, mu_a/mu_t, 1/mu_t, HG terms) precomputed on the host;
+//  - the voxel grid is uint8 in global memory (read through L2), clamped like the reference's
+//    texture (src/mc3d_gpu.cu:416-418);
+//  - the boundary distance uses cached reciprocals of the direction cosines, updated only
+//    when the direction changes;
+//  - the next voxel index comes from the face test of the DDA instead of three floor divisions;
+//  - the HG, roulette and step-length math runs only for lanes that interact, behind a
+//    wave-uniform branch; lanes that only cross a voxel face still advance the stream by the
+//    four draws the reference consumes on every step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/ymc3d.h"
+#include "host_util.h"
+
+namespace ymc {
+
+// ------------------------------------------------------------------------------------------
+// Device helpers.
+
+// MWC step (gpumcml_rng/gpumcml_rng.cu:40) as one v_mad_u64_u32.
+__device__ __forceinline__ uint32_t mwc_next(uint64_t &x, uint32_t a) {
+  x = (uint64_t)(uint32_t)x * (uint64_t)a + (x >> 32);
+  return (uint32_t)x;
+}
+
+// Round-toward-zero u32 -> f32, scaled by 2^-32: rand_MWC_co (gpumcml_rng.cu:41).  Masking the
+// bits below the 24-bit significand makes the conversion exact; the scale is a power of two.
+__device__ __forceinline__ float u32_co(uint32_t u) {
+  uint32_t lz = __clz(u);                         // 32 for u == 0
+  uint32_t sh = lz < 8u ? 8u - lz : 0u;
+  uint32_t m = (u >> sh) << sh;
+  return (float)m * 0x1p-32f;
+}
+__device__ __forceinline__ float u32_oc(uint32_t u) { return 1.0f - u32_co(u); }
+
+__device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
+// Fixed-point weight, scale 2^32, round to nearest even (0 <= w < 2^32 after scaling).
+__device__ __forceinline__ uint64_t fx(float w) {
+  float s = rintf(w * 0x1p32f);
+  return s > 0.0f ? (uint64_t)s : 0ull;
+}
+__device__ __forceinline__ uint64_t fx_sq(uint64_t f) {
+  // (f*f + 2^31) >> 32 with a 128-bit intermediate
+  uint64_t lo = f * f;
+  uint64_t hi = __umul64hi(f, f);
+  uint64_t lo2 = lo + (1ull << 31);
+  hi += (lo2 < lo);
+  return (hi << 32) | (lo2 >> 32);
+}
+
+// photontrack_get_grid_coord (src/mc3d_gpu.cu:53-63), IEEE division; used at launch and tally.
+__device__ __forceinline__ int grid_coord(float x, float ux, float dx) {
+  float q = floorf(x / dx);
+  q = fminf(fmaxf(q, -1073741824.0f), 1073741824.0f);
+  int i = (q == q) ? (int)q : 0;
+  i += (x == (float)(i + 1) * dx);
+  float bmin = (float)i * dx;
+  i = ((x > bmin) || ((x == bmin) && (ux >= 0))) ? i : i - 1;
+  return i;
+}
+
+// Per-row constants in LDS: three float4 per (table, tissue) row.
+//   r0 = {mua, mus, g, n}
+//   r1 = {mu_t, mua/mu_t (0 if mu_t == 0), 1/mu_t, 1/(2g)}
+//   r2 = {1 - g*g, 1 - g, 2*g, 1 + g*g}
+struct RowConst {
+  float4 r0, r1, r2;
+};
+
+struct KArgs {
+  // geometry
+  const uint8_t *vox;
+  int nx, ny, nz;
+  float dx, dy, dz, Lx, Ly;
+  // optical properties
+  const RowConst *rows;  // [num_tables * num_types]
+  const float *w0;       // [num_tables] launch weight 1 - Rsp
+  int num_types, num_tables;
+  // physics / illumination
+  float px, py;
+  float uz_thresh;  // largest float <= 0.99999 (the reference compares in double)
+  float zero_thresh;
+  float w_thresh;   // smallest float >= 1e-4 (W < 1e-4 in double <=> W < w_thresh in float)
+  int fresnel_exact;
+  int kill0;
+  // tally
+  int tnx, tny;
+  float tdx, tdy;
+  int nr;
+  float rdr, rx0, ry0;
+  // Reflectance and transmittance maps share one allocation per quantity: [2][tables][tny][tnx],
+  // R in the first half and T in the second, so a packet's destination is an index offset.
+  unsigned long long *map_w, *map_w2, *Rr_w, *Rr_w2;
+  unsigned int *map_n, *Rr_n;
+  unsigned long long map_half;  // tables * tny * tnx
+  unsigned long long *tot;  // [num_tables][kTotFields]
+  // chains
+  const ymc_chain *chains;
+  const uint64_t *chain_x;
+  const uint32_t *chain_a;
+  const uint64_t *chain_first;  // packet index of each chain's first packet (records only)
+  unsigned long long *next_chain;
+  unsigned long long n_chains;
+  const volatile int *cancel;
+  int poll_interval;
+  ymc_photon_record *records;
+  unsigned long long n_records;
+};
+
+// Offsets into the per-table totals block (ymc_totals field order).
+enum TotField {
+  kPhotons = 0, kRw, kRw2, kTw, kTw2, kRn, kTn, kAw, kRoul, kKilledW, kKilled, kSteps, kFresnel, kTotFields
+};
+static_assert(sizeof(ymc_totals) == kTotFields * 8, "ymc_totals layout");
+
+__device__ __forceinline__ void atomic_add_u64(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
+
+struct Packet {
+  float x, y, z, ux, uy, uz, W, s;
+  float iux, iuy, iuz;
+  int i, j, k, t;
+  uint32_t nstep;  // steps taken by this packet
+};
+
+// Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
+// launch boundary (src/mc3d_gpu.cu:499-504), so its end-of-launch zero-step check (:343) sees a
+// packet's steps 44, 89, ...
+constexpr uint32_t kLaunchSteps = 45;
+// Step cap per packet (a packet that reaches it is killed; see the kill rule in step_packet).
+constexpr uint32_t kMaxSteps = 1u << 24;
+
+__device__ __forceinline__ int voxel_row(const KArgs &a, int i, int j, int k) {
+  int ic = min(max(i, 0), a.nx - 1);
+  int jc = min(max(j, 0), a.ny - 1);
+  int kc = min(max(k, 0), a.nz - 1);
+  return a.vox[((size_t)kc * a.ny + jc) * a.nx + ic];
+}
+
+template <bool UNIFORM>
+__device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float w0) {
+  if (UNIFORM) {
+    p.x = a.Lx * u32_co(mwc_next(rx, ra));  // src/mc3d_gpu.cu:88
+    p.y = a.Ly * u32_co(mwc_next(rx, ra));  // :89
+  } else {
+    p.x = a.px;
+    p.y = a.py;
+  }
+  p.z = 0.0f;
+  p.ux = 0.0f;
+  p.uy = 0.0f;
+  p.uz = 1.0f;
+  p.iux = INFINITY;
+  p.iuy = INFINITY;
+  p.iuz = 1.0f;
+  p.W = w0;                                        // :101
+  p.s = -__logf(u32_oc(mwc_next(rx, ra)));         // :102
+  p.i = grid_coord(p.x, 0.0f, a.dx);
+  p.j = grid_coord(p.y, 0.0f, a.dy);
+  p.k = 0;                                         // grid_coord(0, +1) = 0
+  p.t = voxel_row(a, p.i, p.j, 0);                 // uz > 0: never the air override
+  p.nstep = 0;
+}
+
+struct LaneAcc {
+  uint32_t steps, fresnel, photons;
+  uint64_t absorbed;
+};
+
+// Outcome of one step.
+enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
+
+// One step of a live packet.
+template <bool COUNTERS>
+__device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
+                                            const RowConst *tab, unsigned long long *tot, LaneAcc &acc) {
+  const float4 c0 = tab[p.t].r0;
+  const float4 c1 = tab[p.t].r1;
+  const float ni = c0.w;
+  const float mu_t = c1.x;
+
+  // photontrack_intersection (:29-49) with cached reciprocals.
+  const float bx = (float)(p.ux >= 0.0f ? p.i + 1 : p.i) * a.dx;
+  const float by = (float)(p.uy >= 0.0f ? p.j + 1 : p.j) * a.dy;
+  const float bz = (float)(p.uz >= 0.0f ? p.k + 1 : p.k) * a.dz;
+  const float tx = (bx - p.x) * p.iux;
+  const float ty = (by - p.y) * p.iuy;
+  const float tz = (bz - p.z) * p.iuz;
+  const float db = fminf(tx, fminf(ty, tz));
+
+  const float dbm = db * mu_t;
+  const bool hit = dbm <= p.s;                      // :197
+  const float ds = hit ? dbm : p.s;                 // :198
+  const float hop = hit ? db : ds * c1.z;           // distance moved
+  float nx = (hit && db == tx) ? bx : fmaf(p.ux, hop, p.x);
+  float ny = (hit && db == ty) ? by : fmaf(p.uy, hop, p.y);
+  float nz = (hit && db == tz) ? bz : fmaf(p.uz, hop, p.z);
+  const float s_left = p.s - ds;                    // :207
+
+  // Next voxel (:209-213): an axis steps when the new coordinate reached the face ahead.
+  const int ni_ = p.i + (p.ux >= 0.0f ? (nx >= bx ? 1 : 0) : (nx <= bx ? -1 : 0));
+  const int nj_ = p.j + (p.uy >= 0.0f ? (ny >= by ? 1 : 0) : (ny <= by ? -1 : 0));
+  const int nk_ = p.k + (p.uz >= 0.0f ? (nz >= bz ? 1 : 0) : (nz <= bz ? -1 : 0));
+  int next_t = voxel_row(a, ni_, nj_, nk_);
+  next_t = (nz <= 0.0f && p.uz < 0.0f) ? YMC_TISSUE_AIR : next_t;
+  const float nt = tab[next_t].r0.w;
+
+  float nux = p.ux, nuy = p.uy, nuz = p.uz;
+  bool refl = false;
+  bool turned = !hit;  // direction will change this step
+  if (ni != nt) {      // Fresnel (:226-257)
+    if (COUNTERS) acc.fresnel++;
+    const bool xi = p.i != ni_, yj = p.j != nj_, zk = p.k != nk_;
+    float cos_ai = xi ? p.ux : (yj ? p.uy : p.uz);
+    cos_ai = fminf(fabsf(cos_ai), 1.0f);
+    const float ni_nt = ni / nt;
+    const float sin_ai = sqrtf(fmaxf(1.0f - cos_ai * cos_ai, 0.0f));  // sin(acos(c))
+    const float sin_at = ni_nt * sin_ai;
+    const float cos_at = sqrtf(1.0f - sin_at * sin_at);
+    const float R1 = (sin_ai * cos_at - cos_ai * sin_at) / (sin_ai * cos_at + cos_ai * sin_at);
+    const float r2d = a.fresnel_exact ? (cos_ai * cos_at + sin_ai * sin_at) : (cos_ai * cos_at + sin_at * sin_at);
+    const float R2 = (cos_ai * cos_at - sin_ai * sin_at) / r2d;
+    float R = 0.5f * R1 * R1 * (1.0f + R2 * R2);
+    R = (sin_at > 1.0f) ? 1.0f : R;
+    const float eps = u32_co(mwc_next(rx, ra));
+    refl = eps < R;
+    const float sx = copysignf(cos_at, p.ux), sy = copysignf(cos_at, p.uy), sz = copysignf(cos_at, p.uz);
+    if (hit) {
+      nux = refl ? (xi ? -p.ux : p.ux) : (xi ? sx : p.ux * ni_nt);
+      nuy = refl ? (yj ? -p.uy : p.uy) : ((yj && !xi) ? sy : p.uy * ni_nt);
+      nuz = refl ? (zk ? -p.uz : p.uz) : ((zk && !yj && !xi) ? sz : p.uz * ni_nt);
+    }
+    turned = true;
+  }
+
+  // The four draws of every step: roulette, HG cos, HG phi, new step length (:275-321).
+  const uint32_t u_roul = mwc_next(rx, ra);
+  const uint32_t u_cos = mwc_next(rx, ra);
+  const uint32_t u_phi = mwc_next(rx, ra);
+  const uint32_t u_len = mwc_next(rx, ra);
+
+  float W = p.W;
+  float s_new = s_left;
+  // Interactions: absorption, roulette, scattering, new step length.
+  if (__builtin_amdgcn_ballot_w64(!hit) != 0) {
+    if (!hit) {
+      const float W_in = W;
+      W = W - c1.y * W;  // dW = (mu_a/mu_t)*W (:262-263)
+      if (COUNTERS) acc.absorbed += fx(W_in) - fx(W);
+      if (W < a.w_thresh && mu_t != 0.0f) {  // roulette (:275-277)
+        const float e = u32_co(u_roul);
+        const float Wr = ((double)e < YMC_SURVIVAL_CHANCE) ? (float)((double)W / YMC_SURVIVAL_CHANCE) : 0.0f;
+        if (COUNTERS) atomicAdd(&tot[kRoul], (unsigned long long)((long long)fx(Wr) - (long long)fx(W)));
+        W = Wr;
+      }
+      const float4 c2 = tab[p.t].r2;
+      // HG (:282-294)
+      const float xi = u32_oc(u_cos);
+      float ct = c2.x * rcp(fmaf(c2.z, xi, c2.y));
+      ct = c1.w * (c2.w - ct * ct);
+      const float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+      const float rev = u32_co(u_phi);  // phi / (2 pi)
+      const float sp = __builtin_amdgcn_sinf(rev);
+      const float cp = __builtin_amdgcn_cosf(rev);
+      const float ux = p.ux, uy = p.uy, uz = p.uz;
+      if (fabsf(uz) <= a.uz_thresh) {
+        const float one_m = 1.0f - uz * uz;
+        const float inv = __builtin_amdgcn_rsqf(one_m);
+        const float stv = st * inv;
+        nux = stv * (ux * uz * cp - uy * sp) + ux * ct;
+        nuy = stv * (uy * uz * cp + ux * sp) + uy * ct;
+        nuz = -(one_m * inv) * st * cp + uz * ct;
+      } else {
+        nux = st * cp;
+        nuy = st * sp;
+        nuz = copysignf(ct, ct * uz);
+      }
+      s_new = (W > 0.0f) ? -__logf(u32_oc(u_len)) : 0.0f;  // :319-321
+    }
+  }
+
+  const bool freeze = (W <= 0.0f) || (mu_t == 0.0f);  // :313
+  if (turned && !freeze) {
+    // :302-310
+    nux = (fabsf(nux) < a.zero_thresh) ? 0.0f : nux;
+    nuy = (fabsf(nuy) < a.zero_thresh) ? 0.0f : nuy;
+    nuz = (fabsf(nuz) < a.zero_thresh) ? 0.0f : nuz;
+    const float nrm = __builtin_amdgcn_rsqf(nux * nux + nuy * nuy + nuz * nuz);
+    p.ux = nux * nrm;
+    p.uy = nuy * nrm;
+    p.uz = nuz * nrm;
+    p.iux = rcp(p.ux);
+    p.iuy = rcp(p.uy);
+    p.iuz = rcp(p.uz);
+  }
+  p.W = W;
+  p.s = (freeze || hit) ? s_left : s_new;
+  if (!freeze) {
+    p.x = nx;
+    p.y = ny;
+    p.z = nz;
+  }
+  // Mirror walls (:113-126, :330), with the reference's short-circuit order.
+  if (!refl) {
+    if (p.x >= a.Lx || p.x <= 0.0f) {
+      p.x = (p.x >= a.Lx) ? 2.0f * a.Lx - p.x : p.x;
+      p.x = (p.x <= 0.0f) ? fabsf(p.x) : p.x;
+      p.ux = -p.ux;
+      p.iux = -p.iux;
+      refl = true;
+    } else if (p.y >= a.Ly || p.y <= 0.0f) {
+      p.y = (p.y >= a.Ly) ? 2.0f * a.Ly - p.y : p.y;
+      p.y = (p.y <= 0.0f) ? fabsf(p.y) : p.y;
+      p.uy = -p.uy;
+      p.iuy = -p.iuy;
+      refl = true;
+    }
+  }
+  if (!((hit && refl) || freeze)) {  // :334-339
+    p.i = ni_;
+    p.j = nj_;
+    p.k = nk_;
+    p.t = next_t;
+  }
+  if (COUNTERS) acc.steps++;
+
+  // Kill (:343-353): non-finite state after every step; a zero step length where the reference
+  // checks it (kill0 1: end of each 45-step launch, 2: every step, 0: never).  Two rules the
+  // reference lacks keep every packet finite: a negative step length (the position has left the
+  // voxel its indices name - an interaction rounded onto a mirror wall and the index commit put
+  // the packet outside the grid, after which the wall test bounces it forever) and a step cap.
+  p.nstep++;
+  const bool launch_end = (p.nstep % kLaunchSteps) == 0;
+  const bool zero_check = a.kill0 == 2 || (a.kill0 == 1 && launch_end);
+  const bool bad = !isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z) || !isfinite(p.s) || !isfinite(p.ux) ||
+                   !isfinite(p.uy) || !isfinite(p.uz) || (zero_check && db == 0.0f) || db < 0.0f ||
+                   p.nstep >= kMaxSteps;
+  if (bad) {
+    if (COUNTERS) {
+      atomicAdd(&tot[kKilled], 1ull);
+      atomicAdd(&tot[kKilledW], (unsigned long long)fx(p.W));
+    }
+    p.x = p.y = p.z = 0.0f;
+    p.ux = p.uy = 0.0f;
+    p.uz = 1.0f;
+    p.W = 0.0f;
+    p.s = 0.0f;
+    return kDead;
+  }
+  return freeze ? kFrozen : kLive;
+}
+
+// photon_detector (src/mc3d_photons.cu:151-171) on the tally grid, in fixed point.
+template <bool RADIAL>
+__device__ __forceinline__ void tally_packet(const Packet &p, const KArgs &a, uint32_t table,
+                                             unsigned long long *tot) {
+  const uint64_t f = fx(p.W);
+  if (f == 0) return;
+  const uint64_t f2 = fx_sq(f);
+  const int ix = grid_coord(p.x, p.ux, a.tdx);
+  const int iy = grid_coord(p.y, p.uy, a.tdy);
+  const bool inside = ix >= 0 && iy >= 0 && ix < a.tnx && iy < a.tny;
+  const bool refl = p.z <= 0.0f;  // src/mc3d_photons.cu:413
+  const size_t bin = (refl ? 0 : a.map_half) + ((size_t)table * a.tny + (inside ? iy : 0)) * a.tnx + (inside ? ix : 0);
+  atomicAdd(&tot[refl ? kRw : kTw], (unsigned long long)f);
+  atomicAdd(&tot[refl ? kRw2 : kTw2], (unsigned long long)f2);
+  atomicAdd(&tot[refl ? kRn : kTn], 1ull);
+  if (inside) {
+    atomicAdd(&a.map_w[bin], (unsigned long long)f);
+    atomicAdd(&a.map_w2[bin], (unsigned long long)f2);
+    atomicAdd(&a.map_n[bin], 1u);
+  }
+  if (RADIAL && refl) {
+    const float ddx = p.x - a.rx0, ddy = p.y - a.ry0;
+    const float r = sqrtf(ddx * ddx + ddy * ddy);
+    const float qb = floorf(r / a.rdr);
+    if (qb < (float)a.nr) {
+      const size_t rb = (size_t)table * a.nr + (int)qb;
+      atomicAdd(&a.Rr_w[rb], (unsigned long long)f);
+      atomicAdd(&a.Rr_w2[rb], (unsigned long long)f2);
+      atomicAdd(&a.Rr_n[rb], 1u);
+    }
+  }
+}
+
+constexpr int kBlock = 256;
+
+template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD>
+__global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  RowConst *s_rows = reinterpret_cast<RowConst *>(lds_raw);
+  float *s_w0 = reinterpret_cast<float *>(s_rows + a.num_tables * a.num_types);
+  {
+    const int nrow = a.num_tables * a.num_types;
+    const float4 *src = reinterpret_cast<const float4 *>(a.rows);
+    float4 *dst = reinterpret_cast<float4 *>(s_rows);
+    for (int q = threadIdx.x; q < nrow * 3; q += blockDim.x) dst[q] = src[q];
+    for (int q = threadIdx.x; q < a.num_tables; q += blockDim.x) s_w0[q] = a.w0[q];
+  }
+  __syncthreads();
+
+  const uint32_t lane = __lane_id();
+  Packet p;
+  uint64_t rx = 0;
+  uint32_t ra = 0;
+  uint32_t left = 0, table = 0, done_in_chain = 0;
+  unsigned long long chain = 0;
+  bool alive = false, drained = false, stop = false;
+  LaneAcc acc{0, 0, 0, 0};
+  uint32_t iter = 0;
+
+  for (;;) {
+    // Cancel flag (host memory), read by the first lane of the wave every poll_interval steps.
+    if (a.cancel && (iter % (uint32_t)a.poll_interval) == 0) {
+      int c = 0;
+      if (lane == (uint32_t)__builtin_amdgcn_readfirstlane(lane)) c = *a.cancel;
+      c = __shfl(c, __builtin_amdgcn_readfirstlane(lane));
+      stop = stop || (c != 0);
+    }
+    ++iter;
+    if (!alive && !stop) {
+      if (left == 0 && !drained) {
+        // Chain finished: flush its counters, then take the next chain.  One atomic per wave.
+        if (COUNTERS && acc.photons) {
+          unsigned long long *tt = a.tot + (size_t)table * kTotFields;
+          atomicAdd(&tt[kPhotons], (unsigned long long)acc.photons);
+          atomicAdd(&tt[kSteps], (unsigned long long)acc.steps);
+          atomicAdd(&tt[kFresnel], (unsigned long long)acc.fresnel);
+          atomicAdd(&tt[kAw], (unsigned long long)acc.absorbed);
+          acc = LaneAcc{0, 0, 0, 0};
+        } else if (!COUNTERS && acc.photons) {
+          atomicAdd(&a.tot[(size_t)table * kTotFields + kPhotons], (unsigned long long)acc.photons);
+          acc.photons = 0;
+        }
+        const uint64_t want = __ballot(1);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+        const uint32_t leader = __builtin_amdgcn_readfirstlane(lane);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.next_chain, (unsigned long long)__popcll(want));
+        base = __shfl(base, leader);
+        const unsigned long long c = base + rank;
+        if (c < a.n_chains) {
+          chain = c;
+          const ymc_chain ch = a.chains[c];
+          left = ch.count;
+          table = ch.table;
+          rx = a.chain_x[c];
+          ra = a.chain_a[c];
+          done_in_chain = 0;
+        } else {
+          drained = true;
+        }
+      }
+      if (left > 0) {
+        launch_packet<UNIFORM>(p, rx, ra, a, s_w0[table]);
+        alive = true;
+      }
+    }
+    if (__ballot(alive) == 0) break;
+    if (alive) {
+      const RowConst *tab = s_rows + (size_t)table * a.num_types;
+      unsigned long long *tt = a.tot + (size_t)table * kTotFields;
+      const int res = step_packet<COUNTERS>(p, rx, ra, a, tab, tt, acc);
+      if (res != kLive) {
+        tally_packet<RADIAL>(p, a, table, tt);
+        if (RECORD) {
+          const unsigned long long gi = a.chain_first[chain] + done_in_chain;
+          if (gi < a.n_records) {
+            ymc_photon_record r;
+            r.x = p.x;
+            r.y = p.y;
+            r.z = p.z;
+            r.w = p.W;
+            r.steps = p.nstep;
+            r.fate = res == kDead ? 2 : (p.z <= 0.0f ? 0 : 1);
+            a.records[gi] = r;
+          }
+        }
+        alive = false;
+        --left;
+        ++done_in_chain;
+        acc.photons++;
+      }
+    }
+  }
+  // Final flush.
+  if (acc.photons) {
+    unsigned long long *tt = a.tot + (size_t)table * kTotFields;
+    atomicAdd(&tt[kPhotons], (unsigned long long)acc.photons);
+    if (COUNTERS) {
+      atomicAdd(&tt[kSteps], (unsigned long long)acc.steps);
+      atomicAdd(&tt[kFresnel], (unsigned long long)acc.fresnel);
+      atomicAdd(&tt[kAw], (unsigned long long)acc.absorbed);
+    }
+  }
+}
+
+// RNG test hook: n_streams lanes, n_draws draws each.
+__global__ void rng_fill_kernel(const uint64_t *x, const uint32_t *a, uint32_t n, uint32_t draws, int kind,
+                                float *out, uint64_t *xf) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint64_t st = x[s];
+  const uint32_t m = a[s];
+  for (uint32_t d = 0; d < draws; ++d) {
+    const uint32_t u = mwc_next(st, m);
+    out[(size_t)d * n + s] = kind ? u32_oc(u) : u32_co(u);
+  }
+  xf[s] = st;
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side.
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t err_ = (expr);                                                                 \
+    if (err_ != hipSuccess)                                                                   \
+      return fail(YMC_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(err_), __FILE__, __LINE__); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ymc_status ensure(size_t bytes) {
+    if (bytes <= n && p) return YMC_OK;
+    release();
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      return fail(YMC_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    n = bytes;
+    return YMC_OK;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+}  // namespace ymc
+
+struct ymc_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 0;
+  std::vector<uint32_t> table;
+  // geometry
+  ymc::DevBuf vox;
+  int nx = 0, ny = 0, nz = 0;
+  float dx = 0, dy = 0, dz = 0;
+  bool have_geom = false;
+  // optprops
+  ymc::DevBuf rows, w0;
+  int num_types = 0, num_tables = 0;
+  bool have_props = false;
+  std::vector<float> host_n;  // for validation
+  // prepared run
+  ymc::DevBuf chains, chain_x, chain_a, chain_first;
+  uint64_t n_chains = 0;
+  uint64_t n_packets = 0;
+  ymc_params params{};
+  bool prepared = false;
+  // tallies
+  ymc::DevBuf map_w, map_w2, map_n, Rr_w, Rr_w2, Rr_n, tot, counter, records;
+  int tnx = 0, tny = 0, nr = 0;
+  float tdx = 0, tdy = 0;
+  uint64_t n_records = 0;
+  double last_ms = 0;
+  int last_pool = 0;
+  // Cancel flag the kernel polls: pinned, mapped host memory.  ymc_engine_run copies the
+  // caller's flag (any host pointer) into it while the kernel runs.
+  int *cancel_host = nullptr;
+  int *cancel_dev = nullptr;
+};
+
+namespace ymc {
+
+static float largest_float_le(double v) {
+  float f = (float)v;
+  if ((double)f > v) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+static float smallest_float_ge(double v) {
+  float f = (float)v;
+  if ((double)f < v) f = std::nextafter(f, INFINITY);
+  return f;
+}
+
+static ymc_status check_engine(ymc_engine *e) {
+  if (!e) return fail(YMC_ERR_ARG, "null engine");
+  if (hipSetDevice(e->device) != hipSuccess) return fail(YMC_ERR_DEVICE, "hipSetDevice(%d) failed", e->device);
+  return YMC_OK;
+}
+
+template <bool U, bool C, bool R, bool REC>
+static const void *kernel_ptr() {
+  return reinterpret_cast<const void *>(&transport_kernel<U, C, R, REC>);
+}
+
+static const void *select_kernel(bool uniform, bool counters, bool radial, bool record) {
+  int key = (uniform ? 8 : 0) | (counters ? 4 : 0) | (radial ? 2 : 0) | (record ? 1 : 0);
+  switch (key) {
+    case 0: return kernel_ptr<false, false, false, false>();
+    case 1: return kernel_ptr<false, false, false, true>();
+    case 2: return kernel_ptr<false, false, true, false>();
+    case 3: return kernel_ptr<false, false, true, true>();
+    case 4: return kernel_ptr<false, true, false, false>();
+    case 5: return kernel_ptr<false, true, false, true>();
+    case 6: return kernel_ptr<false, true, true, false>();
+    case 7: return kernel_ptr<false, true, true, true>();
+    case 8: return kernel_ptr<true, false, false, false>();
+    case 9: return kernel_ptr<true, false, false, true>();
+    case 10: return kernel_ptr<true, false, true, false>();
+    case 11: return kernel_ptr<true, false, true, true>();
+    case 12: return kernel_ptr<true, true, false, false>();
+    case 13: return kernel_ptr<true, true, false, true>();
+    case 14: return kernel_ptr<true, true, true, false>();
+    default: return kernel_ptr<true, true, true, true>();
+  }
+}
+
+}  // namespace ymc
+
+using namespace ymc;
+
+extern "C" {
+
+ymc_status ymc_engine_create(int32_t device, const uint32_t *table, uint32_t table_len, ymc_engine **out) {
+  if (!out || !table || table_len < 2) return fail(YMC_ERR_ARG, "ymc_engine_create: need a multiplier table of >= 2 entries");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(YMC_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(YMC_ERR_ARG, "device %d out of range (have %d)", device, ndev);
+  ymc_engine *e = new ymc_engine();
+  e->device = device;
+  e->table.assign(table, table + table_len);
+  hipError_t st = hipSetDevice(device);
+  if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (st == hipSuccess) st = hipEventCreate(&e->ev0);
+  if (st == hipSuccess) st = hipEventCreate(&e->ev1);
+  if (st == hipSuccess) st = hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (st != hipSuccess) {
+    ymc_status r = fail(YMC_ERR_DEVICE, "engine setup failed: %s", hipGetErrorString(st));
+    ymc_engine_destroy(e);
+    return r;
+  }
+  if (e->counter.ensure(64) != YMC_OK || e->tot.ensure(sizeof(ymc_totals))) {
+    ymc_engine_destroy(e);
+    return fail(YMC_ERR_NOMEM, "engine allocation failed");
+  }
+  st = hipHostMalloc(reinterpret_cast<void **>(&e->cancel_host), sizeof(int), hipHostMallocMapped | hipHostMallocCoherent);
+  if (st == hipSuccess) st = hipHostGetDevicePointer(reinterpret_cast<void **>(&e->cancel_dev), e->cancel_host, 0);
+  if (st != hipSuccess) {
+    ymc_status r = fail(YMC_ERR_DEVICE, "pinned cancel flag: %s", hipGetErrorString(st));
+    ymc_engine_destroy(e);
+    return r;
+  }
+  *e->cancel_host = 0;
+  *out = e;
+  return YMC_OK;
+}
+
+void ymc_engine_destroy(ymc_engine *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->cancel_host) (void)hipHostFree(e->cancel_host);
+  delete e;  // DevBuf destructors free device memory
+}
+
+ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!g || !g->tissue || g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || !(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0))
+    return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: invalid geometry");
+  size_t nv = (size_t)g->nx * g->ny * g->nz;
+  if ((st = e->vox.ensure(nv))) return st;
+  HIP_TRY(hipMemcpy(e->vox.p, g->tissue, nv, hipMemcpyHostToDevice));
+  e->nx = g->nx;
+  e->ny = g->ny;
+  e->nz = g->nz;
+  e->dx = g->dx;
+  e->dy = g->dy;
+  e->dz = g->dz;
+  // remember the largest row for validation against the property tables
+  uint8_t mx = 0;
+  for (size_t v = 0; v < nv; ++v) mx = std::max(mx, g->tissue[v]);
+  e->host_n.assign(1, (float)mx);
+  e->have_geom = true;
+  e->prepared = false;
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_set_optprops(ymc_engine *e, const ymc_optprops *p) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!p || p->num_types < 2 || p->num_tables < 1 || !p->mua || !p->mus || !p->g || !p->n)
+    return fail(YMC_ERR_ARG, "ymc_engine_set_optprops: need >= 2 rows (ambient + tissue) and >= 1 table");
+  const int T = p->num_types, L = p->num_tables;
+  std::vector<RowConst> rows((size_t)T * L);
+  std::vector<float> w0(L);
+  for (int w = 0; w < L; ++w) {
+    for (int t = 0; t < T; ++t) {
+      size_t q = (size_t)w * T + t;
+      float mua = p->mua[q], mus = p->mus[q], g = p->g[q], n = p->n[q];
+      if (!(mua >= 0) || !(mus >= 0) || !(n > 0) || !(g >= -1 && g <= 1))
+        return fail(YMC_ERR_ARG, "optical properties of table %d row %d out of range", w, t);
+      float mut = mua + mus;
+      RowConst &r = rows[q];
+      r.r0 = make_float4(mua, mus, g, n);
+      r.r1 = make_float4(mut, mut > 0 ? mua / mut : 0.0f, mut > 0 ? 1.0f / mut : 0.0f, 1.0f / (2 * g));
+      r.r2 = make_float4(1 - g * g, 1 - g, 2 * g, 1 + g * g);
+    }
+    // photontrack_calc_specular_reflection(n[air], n[1]) (src/mc3d_gpu.cu:67-70, :500)
+    float n0 = p->n[(size_t)w * T + YMC_TISSUE_AIR], n1 = p->n[(size_t)w * T + 1];
+    float r = (n0 - n1) / (n0 + n1);
+    w0[w] = 1.0f - r * r;
+  }
+  if ((st = e->rows.ensure(rows.size() * sizeof(RowConst)))) return st;
+  if ((st = e->w0.ensure(w0.size() * sizeof(float)))) return st;
+  HIP_TRY(hipMemcpy(e->rows.p, rows.data(), rows.size() * sizeof(RowConst), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->w0.p, w0.data(), w0.size() * sizeof(float), hipMemcpyHostToDevice));
+  e->num_types = T;
+  e->num_tables = L;
+  e->have_props = true;
+  e->prepared = false;
+  return YMC_OK;
+}
+
+static ymc_status prepare_common(ymc_engine *e, const ymc_params *prm) {
+  if (!e->have_geom || !e->have_props) return fail(YMC_ERR_STATE, "set geometry and optical properties first");
+  if (!prm) return fail(YMC_ERR_ARG, "null params");
+  if (!e->host_n.empty() && e->host_n[0] >= (float)e->num_types)
+    return fail(YMC_ERR_FORMAT, "geometry uses row %d but the tables have %d rows", (int)e->host_n[0], e->num_types);
+  e->params = *prm;
+  e->tnx = prm->tally_nx > 0 ? prm->tally_nx : e->nx;
+  e->tny = prm->tally_ny > 0 ? prm->tally_ny : e->ny;
+  e->tdx = prm->tally_dx > 0 ? prm->tally_dx : e->dx;
+  e->tdy = prm->tally_dy > 0 ? prm->tally_dy : e->dy;
+  e->nr = prm->radial_bins > 0 ? prm->radial_bins : 0;
+  if (e->nr > 0 && !(prm->radial_dr > 0)) return fail(YMC_ERR_ARG, "radial_dr must be > 0");
+  const size_t L = e->num_tables;
+  const size_t mapn = L * e->tnx * e->tny;
+  ymc_status st;
+  if ((st = e->map_w.ensure(2 * mapn * 8)) || (st = e->map_w2.ensure(2 * mapn * 8)) ||
+      (st = e->map_n.ensure(2 * mapn * 4)) || (st = e->Rr_w.ensure(L * e->nr * 8)) || (st = e->Rr_w2.ensure(L * e->nr * 8)) ||
+      (st = e->Rr_n.ensure(L * e->nr * 4)) || (st = e->tot.ensure(L * sizeof(ymc_totals))))
+    return st;
+  return YMC_OK;
+}
+
+static ymc_status upload_chains(ymc_engine *e, const ymc_chain *chains, const uint64_t *x, const uint32_t *a,
+                                uint64_t n) {
+  ymc_status st;
+  if ((st = e->chains.ensure(n * sizeof(ymc_chain))) || (st = e->chain_x.ensure(n * 8)) ||
+      (st = e->chain_a.ensure(n * 4)))
+    return st;
+  HIP_TRY(hipMemcpy(e->chains.p, chains, n * sizeof(ymc_chain), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->chain_x.p, x, n * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->chain_a.p, a, n * 4, hipMemcpyHostToDevice));
+  uint64_t total = 0;
+  std::vector<uint64_t> first(n);
+  for (uint64_t c = 0; c < n; ++c) {
+    if (chains[c].table >= (uint32_t)e->num_tables) return fail(YMC_ERR_ARG, "chain %llu: table out of range", (unsigned long long)c);
+    first[c] = total;
+    total += chains[c].count;
+  }
+  if (e->params.flags & YMC_FLAG_RECORD) {
+    if ((st = e->chain_first.ensure(n * 8)) || (st = e->records.ensure(total * sizeof(ymc_photon_record)))) return st;
+    HIP_TRY(hipMemcpy(e->chain_first.p, first.data(), n * 8, hipMemcpyHostToDevice));
+    e->n_records = total;
+  } else {
+    e->n_records = 0;
+  }
+  e->n_chains = n;
+  e->n_packets = total;
+  e->prepared = true;
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_prepare(ymc_engine *e, const ymc_params *prm) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if ((st = prepare_common(e, prm))) return st;
+  const int L = e->num_tables;
+  std::vector<uint64_t> budgets(L);
+  for (int w = 0; w < L; ++w) budgets[w] = prm->budgets ? prm->budgets[w] : prm->num_photons;
+  const int lanes = prm->chain_lanes > 0 ? prm->chain_lanes : 524288;
+  uint64_t n = 0;
+  if ((st = build_chains(budgets.data(), L, lanes, nullptr, 0, &n))) return st;
+  std::vector<ymc_chain> ch(n);
+  if ((st = build_chains(budgets.data(), L, lanes, ch.data(), n, &n))) return st;
+  std::vector<uint64_t> x(n);
+  std::vector<uint32_t> a(n);
+  if ((st = seed_streams(e->table.data(), (uint32_t)e->table.size(), prm->seed ? prm->seed : 1, n, x.data(), a.data())))
+    return st;
+  return upload_chains(e, ch.data(), x.data(), a.data(), n);
+}
+
+ymc_status ymc_engine_prepare_chains(ymc_engine *e, const ymc_params *prm, const ymc_chain *chains, const uint64_t *x,
+                                     const uint32_t *a, uint64_t n_chains) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!chains || !x || !a) return fail(YMC_ERR_ARG, "ymc_engine_prepare_chains: null chain arrays");
+  if ((st = prepare_common(e, prm))) return st;
+  return upload_chains(e, chains, x, a, n_chains);
+}
+
+ymc_status ymc_engine_run(ymc_engine *e) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!e->prepared) return fail(YMC_ERR_STATE, "ymc_engine_run: call ymc_engine_prepare first");
+  const ymc_params &prm = e->params;
+  const size_t L = e->num_tables;
+  const size_t mapn = L * e->tnx * e->tny;
+  hipStream_t s = e->stream;
+  HIP_TRY(hipMemsetAsync(e->map_w.p, 0, 2 * mapn * 8, s));
+  HIP_TRY(hipMemsetAsync(e->map_w2.p, 0, 2 * mapn * 8, s));
+  HIP_TRY(hipMemsetAsync(e->map_n.p, 0, 2 * mapn * 4, s));
+  if (e->nr) {
+    HIP_TRY(hipMemsetAsync(e->Rr_w.p, 0, L * e->nr * 8, s));
+    HIP_TRY(hipMemsetAsync(e->Rr_w2.p, 0, L * e->nr * 8, s));
+    HIP_TRY(hipMemsetAsync(e->Rr_n.p, 0, L * e->nr * 4, s));
+  }
+  HIP_TRY(hipMemsetAsync(e->tot.p, 0, L * sizeof(ymc_totals), s));
+  HIP_TRY(hipMemsetAsync(e->counter.p, 0, 8, s));
+  if (e->n_records) HIP_TRY(hipMemsetAsync(e->records.p, 0, e->n_records * sizeof(ymc_photon_record), s));
+
+  KArgs a{};
+  a.vox = e->vox.as<uint8_t>();
+  a.nx = e->nx;
+  a.ny = e->ny;
+  a.nz = e->nz;
+  a.dx = e->dx;
+  a.dy = e->dy;
+  a.dz = e->dz;
+  a.Lx = e->dx * (float)e->nx;
+  a.Ly = e->dy * (float)e->ny;
+  a.rows = e->rows.as<RowConst>();
+  a.w0 = e->w0.as<float>();
+  a.num_types = e->num_types;
+  a.num_tables = e->num_tables;
+  a.px = prm.pencil_x >= 0 ? prm.pencil_x : a.Lx / 2;
+  a.py = prm.pencil_y >= 0 ? prm.pencil_y : a.Ly / 2;
+  a.uz_thresh = largest_float_le(YMC_UZ_THRESH);
+  a.zero_thresh = (float)(1 - YMC_UZ_THRESH);
+  a.w_thresh = smallest_float_ge(YMC_W_THRESHOLD);
+  a.fresnel_exact = prm.fresnel_mode == YMC_FRESNEL_EXACT;
+  a.kill0 = prm.kill_zero_step;
+  a.tnx = e->tnx;
+  a.tny = e->tny;
+  a.tdx = e->tdx;
+  a.tdy = e->tdy;
+  a.nr = e->nr;
+  a.rdr = prm.radial_dr;
+  a.rx0 = prm.radial_x0 >= 0 ? prm.radial_x0 : a.px;
+  a.ry0 = prm.radial_y0 >= 0 ? prm.radial_y0 : a.py;
+  a.map_w = e->map_w.as<unsigned long long>();
+  a.map_w2 = e->map_w2.as<unsigned long long>();
+  a.map_n = e->map_n.as<unsigned int>();
+  a.map_half = mapn;
+  a.Rr_w = e->Rr_w.as<unsigned long long>();
+  a.Rr_w2 = e->Rr_w2.as<unsigned long long>();
+  a.Rr_n = e->Rr_n.as<unsigned int>();
+  a.tot = e->tot.as<unsigned long long>();
+  a.chains = e->chains.as<ymc_chain>();
+  a.chain_x = e->chain_x.as<uint64_t>();
+  a.chain_a = e->chain_a.as<uint32_t>();
+  a.chain_first = e->chain_first.as<uint64_t>();
+  a.next_chain = e->counter.as<unsigned long long>();
+  a.n_chains = e->n_chains;
+  *e->cancel_host = (prm.cancel && *prm.cancel) ? 1 : 0;
+  a.cancel = e->cancel_dev;
+  a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
+  a.records = e->records.as<ymc_photon_record>();
+  a.n_records = e->n_records;
+
+  const bool uniform = prm.illumination == YMC_ILLUM_UNIFORM;
+  const bool counters = (prm.flags & YMC_FLAG_COUNTERS) != 0;
+  const bool record = (prm.flags & YMC_FLAG_RECORD) != 0;
+  const void *kern = select_kernel(uniform, counters, e->nr > 0, record);
+  const size_t lds = (size_t)e->num_tables * e->num_types * sizeof(RowConst) + e->num_tables * sizeof(float);
+  if (lds > 64 * 1024) return fail(YMC_ERR_ARG, "optical-property tables need %zu B of LDS (max 65536)", lds);
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds));
+  if (per_cu < 1) per_cu = 1;
+  int blocks = per_cu * e->num_cus;
+  if (prm.pool_lanes > 0) blocks = std::max(1, prm.pool_lanes / kBlock);
+  e->last_pool = blocks * kBlock;
+  void *args[] = {&a};
+  HIP_TRY(hipEventRecord(e->ev0, s));
+  HIP_TRY(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, lds, s));
+  HIP_TRY(hipEventRecord(e->ev1, s));
+  // Wait for the kernel, forwarding the caller's cancel flag to the pinned copy it polls.
+  for (;;) {
+    hipError_t q = hipStreamQuery(s);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) return fail(YMC_ERR_DEVICE, "transport kernel failed: %s", hipGetErrorString(q));
+    if (prm.cancel && *prm.cancel && !*e->cancel_host) *e->cancel_host = 1;
+    std::this_thread::sleep_for(std::chrono::microseconds(100));
+  }
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  e->last_ms = ms;
+  if (prm.cancel && *prm.cancel) return fail(YMC_ERR_CANCELLED, "run cancelled; tallies hold the finished packets");
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_read_tally(ymc_engine *e, ymc_tally *out) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!out) return fail(YMC_ERR_ARG, "null tally");
+  if (!e->prepared) return fail(YMC_ERR_STATE, "no run prepared");
+  const size_t L = e->num_tables;
+  const size_t mapn = L * e->tnx * e->tny;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  auto cp = [&](void *dst, const DevBuf &src, size_t bytes, size_t offset = 0) -> hipError_t {
+    if (!dst || bytes == 0) return hipSuccess;
+    return hipMemcpy(dst, static_cast<const char *>(src.p) + offset, bytes, hipMemcpyDeviceToHost);
+  };
+  HIP_TRY(cp(out->R_w, e->map_w, mapn * 8));
+  HIP_TRY(cp(out->R_w2, e->map_w2, mapn * 8));
+  HIP_TRY(cp(out->R_n, e->map_n, mapn * 4));
+  HIP_TRY(cp(out->T_w, e->map_w, mapn * 8, mapn * 8));
+  HIP_TRY(cp(out->T_w2, e->map_w2, mapn * 8, mapn * 8));
+  HIP_TRY(cp(out->T_n, e->map_n, mapn * 4, mapn * 4));
+  HIP_TRY(cp(out->Rr_w, e->Rr_w, L * e->nr * 8));
+  HIP_TRY(cp(out->Rr_w2, e->Rr_w2, L * e->nr * 8));
+  HIP_TRY(cp(out->Rr_n, e->Rr_n, L * e->nr * 4));
+  HIP_TRY(cp(out->totals, e->tot, L * sizeof(ymc_totals)));
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_read_records(ymc_engine *e, ymc_photon_record *out, uint64_t cap) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!out) return fail(YMC_ERR_ARG, "null output");
+  if (!e->n_records) return fail(YMC_ERR_STATE, "last run did not record packets (YMC_FLAG_RECORD)");
+  uint64_t n = std::min(cap, e->n_records);
+  HIP_TRY(hipMemcpy(out, e->records.p, n * sizeof(ymc_photon_record), hipMemcpyDeviceToHost));
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_last_kernel_ms(ymc_engine *e, double *ms) {
+  if (!e || !ms) return fail(YMC_ERR_ARG, "null argument");
+  *ms = e->last_ms;
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_pool_lanes(ymc_engine *e, int32_t *lanes) {
+  if (!e || !lanes) return fail(YMC_ERR_ARG, "null argument");
+  *lanes = e->last_pool;
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t *a, uint32_t n_streams,
+                               uint32_t n_draws, int32_t kind, float *out, uint64_t *x_final) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!x || !a || !out || !x_final || n_streams == 0) return fail(YMC_ERR_ARG, "ymc_engine_rng_fill: bad arguments");
+  DevBuf dx, da, dout, dxf;
+  if ((st = dx.ensure((size_t)n_streams * 8)) || (st = da.ensure((size_t)n_streams * 4)) ||
+      (st = dout.ensure((size_t)n_streams * n_draws * 4 + 4)) || (st = dxf.ensure((size_t)n_streams * 8)))
+    return st;
+  HIP_TRY(hipMemcpy(dx.p, x, (size_t)n_streams * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(da.p, a, (size_t)n_streams * 4, hipMemcpyHostToDevice));
+  const uint32_t blk = 256, grid = (n_streams + blk - 1) / blk;
+  hipLaunchKernelGGL(rng_fill_kernel, dim3(grid), dim3(blk), 0, e->stream, dx.as<uint64_t>(), da.as<uint32_t>(),
+                     n_streams, n_draws, kind, dout.as<float>(), dxf.as<uint64_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipMemcpy(out, dout.p, (size_t)n_streams * n_draws * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(x_final, dxf.p, (size_t)n_streams * 8, hipMemcpyDeviceToHost));
+  return YMC_OK;
+}
+
+ymc_status ymc_run_3dmc_gpu(int32_t nx, int32_t ny, int32_t nz, float dx, float dy, float dz, const int32_t *tissue,
+                            int32_t num_types, const float *mua, const float *mus, const float *g, const float *n,
+                            int32_t num_photons, uint64_t seed, const uint32_t *table, uint32_t table_len, double *R,
+                            double *R_tot, double *T, double *T_tot, float *A, int32_t *num_finished) {
+  (void)A;
+  if (!tissue || !R || !T || !R_tot || !T_tot || !num_finished || num_photons < 0)
+    return fail(YMC_ERR_ARG, "ymc_run_3dmc_gpu: null argument");
+  size_t nv = (size_t)nx * ny * nz;
+  std::vector<uint8_t> vox(nv);
+  for (size_t v = 0; v < nv; ++v) {
+    if (tissue[v] < 0 || tissue[v] >= num_types || tissue[v] > 255)
+      return fail(YMC_ERR_FORMAT, "voxel %zu: row %d outside the optical-property table", v, tissue[v]);
+    vox[v] = (uint8_t)tissue[v];
+  }
+  ymc_engine *e = nullptr;
+  ymc_status st = ymc_engine_create(0, table, table_len, &e);
+  if (st) return st;
+  struct Guard {
+    ymc_engine *e;
+    ~Guard() { ymc_engine_destroy(e); }
+  } guard{e};
+  ymc_geometry geo{nx, ny, nz, dx, dy, dz, vox.data()};
+  ymc_optprops op{num_types, 1, mua, mus, g, n};
+  if ((st = ymc_engine_set_geometry(e, &geo)) || (st = ymc_engine_set_optprops(e, &op))) return st;
+  ymc_params prm{};
+  prm.num_photons = (uint64_t)num_photons;
+  prm.seed = seed;
+  prm.illumination = YMC_ILLUM_UNIFORM;
+  prm.pencil_x = prm.pencil_y = -1;
+  prm.radial_x0 = prm.radial_y0 = -1;
+  prm.kill_zero_step = 1;
+  if ((st = ymc_engine_prepare(e, &prm)) || (st = ymc_engine_run(e))) return st;
+  std::vector<uint64_t> rw(nv / nz), tw(nv / nz);
+  ymc_totals tt{};
+  ymc_tally out{};
+  out.R_w = rw.data();
+  out.T_w = tw.data();
+  out.totals = &tt;
+  if ((st = ymc_engine_read_tally(e, &out))) return st;
+  for (size_t b = 0; b < rw.size(); ++b) {
+    R[b] += (double)rw[b] / YMC_FX_ONE;  // accumulate like the reference's tally
+    T[b] += (double)tw[b] / YMC_FX_ONE;
+  }
+  *R_tot = (double)tt.R_w / YMC_FX_ONE;
+  *T_tot = (double)tt.T_w / YMC_FX_ONE;
+  *num_finished = (int32_t)tt.photons;
+  return YMC_OK;
+}
+
+}  // extern "C"
