@@ -101,6 +101,9 @@ typedef struct ymc_params {
   int32_t pool_lanes;          /* resident lanes (device); 0 selects full occupancy */
   int32_t compact_interval;    /* steps between pool compaction points; 0 selects default */
   uint32_t flags;              /* YMC_FLAG_* */
+  int32_t fast_min_lanes;      /* fast-crossing loop runs while >= this many lanes of a wave can
+                                  take a plain voxel crossing; 0 selects default, >64 disables */
+  int32_t fast_max_iters;      /* cap on consecutive fast-crossing iterations; 0 selects default */
 } ymc_params;
 
 /* Per-table totals (fixed point, scale YMC_FX_ONE). */
@@ -183,6 +186,12 @@ ymc_status ymc_write_optprops(const char *path, int32_t num_records, const doubl
 ymc_status ymc_write_property(const char *path, int32_t dims, int32_t nx, int32_t ny, int32_t nz,
                               double dx, double dy, double dz, const double *data);
 void ymc_free(void *p);
+
+/* Safe radius per voxel (host): the largest r <= 31 such that every voxel within Chebyshev
+ * distance r has the same row, lies inside the lateral grid and not above the top layer (below
+ * the grid the bottom layer repeats).  The engine packs it into its voxel words; exported as a
+ * test hook.  No counterpart in the reference. */
+ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *out);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Device engine.                                                                             */

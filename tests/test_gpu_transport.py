@@ -192,3 +192,40 @@ def test_run_3dmc_gpu_dropin(multipliers):
     assert nf == 20000
     assert R.shape == (64, 64) and abs(R.sum() - Rtot) < 1e-6 * max(Rtot, 1)
     assert 0.02 < Rtot / nf < 0.6
+
+
+@pytest.mark.parametrize("scene_fn,n", [(configs.skin_vessels, 200_000), (configs.skin_slab_sphere, 100_000)])
+def test_fast_crossings_change_nothing(engine, multipliers, scene_fn, n):
+    """Plain crossings taken in the inner safe-cube loop are the general step's crossings: the
+    tallies and counters are bit-identical with the loop disabled."""
+    scene = scene_fn(photons=n)
+    scene.load(engine)
+    base = dict(scene.params, flags=y.FLAG_COUNTERS)
+    slow = engine.simulate(**base, fast_min_lanes=65)
+    fast = engine.simulate(**base)
+    for f in ("R_w", "R_w2", "R_n", "T_w", "T_w2", "T_n"):
+        assert np.array_equal(getattr(slow, f), getattr(fast, f)), f
+    assert slow.totals.tobytes() == fast.totals.tobytes()
+    print(f"{scene.name}: kernel {slow.kernel_ms:.1f} ms without, {fast.kernel_ms:.1f} ms with fast crossings")
+
+
+def test_cli_end_to_end(tmp_path, multipliers):
+    """ymc3d geometry.bin optprops.bid definition outname, as the reference's gpu-mc."""
+    import os
+    import subprocess
+
+    from ymc3d_amd import io
+
+    scene = configs.skin_slab_sphere()
+    geo, props = tmp_path / "geometry.bin", tmp_path / "optprops.bid"
+    io.write_geometry(str(geo), scene.file_values, *scene.voxel)
+    io.write_optprops(str(props), scene.records[0])
+    (tmp_path / "def.txt").write_text("photons 50000\nseed 7\nillumination pencil\n")
+    exe = os.path.join(os.path.dirname(y._abi.LIB_PATH), "bin", "ymc3d")
+    out = subprocess.run([exe, str(geo), str(props), str(tmp_path / "def.txt"), str(tmp_path / "run")],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    drs, d = io.read_property(str(tmp_path / "run_drs.bin"))
+    assert drs.shape == (64, 64) and d == pytest.approx((0.01, 0.01))
+    assert (tmp_path / "run_simparam.dat").read_text().startswith("50000 #num_photons")
+    assert 0.1 < drs.sum() < 0.3

@@ -41,7 +41,8 @@ class Params(C.Structure):
                 ("radial_x0", C.c_float), ("radial_y0", C.c_float),
                 ("fresnel_mode", C.c_int32), ("kill_zero_step", C.c_int32),
                 ("chain_lanes", C.c_int32), ("pool_lanes", C.c_int32),
-                ("compact_interval", C.c_int32), ("flags", C.c_uint32)]
+                ("compact_interval", C.c_int32), ("flags", C.c_uint32),
+                ("fast_min_lanes", C.c_int32), ("fast_max_iters", C.c_int32)]
 
 
 class Totals(C.Structure):
@@ -111,6 +112,7 @@ _SIGS = {
     "ymc_write_property": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                      C.c_double, C.c_double, P(C.c_double)]),
     "ymc_free": (None, [C.c_void_p]),
+    "ymc_safe_radius": (C.c_int, [P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8)]),
     "ymc_engine_create": (C.c_int, [C.c_int32, P(C.c_uint32), C.c_uint32, P(C.c_void_p)]),
     "ymc_engine_destroy": (None, [C.c_void_p]),
     "ymc_engine_set_geometry": (C.c_int, [C.c_void_p, P(Geometry)]),

@@ -232,6 +232,74 @@ ymc_status build_chains(const uint64_t *budgets, int32_t num_tables, int32_t cha
 }
 
 // ------------------------------------------------------------------------------------------
+// Safe radius of every voxel: the largest r <= kSafeMax such that every voxel within Chebyshev
+// distance r has the same row, lies inside the lateral grid (not beyond a mirror wall) and not
+// above the top layer.  Below the grid the reference's texture clamps to the bottom layer
+// (src/mc3d_gpu.cu:416-418), so the bottom neighbourhood is read through the same clamp.
+// S_1(v): all 26 neighbours have v's row; S_r = erosion of S_{r-1} by the 3x3x3 cube, done as
+// three separable 1-D erosions.
+
+namespace {
+void erode_axis(const uint8_t *in, uint8_t *out, int nx, int ny, int nz, int axis) {
+  auto idx = [&](int i, int j, int k) { return ((size_t)k * ny + j) * nx + i; };
+  unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nthreads; ++t) {
+    pool.emplace_back([&, t] {
+      for (int k = (int)t; k < nz; k += (int)nthreads)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) {
+            uint8_t v = in[idx(i, j, k)];
+            if (v) {
+              if (axis == 0) {
+                v = (i > 0 && i < nx - 1) ? (in[idx(i - 1, j, k)] & in[idx(i + 1, j, k)]) : 0;
+              } else if (axis == 1) {
+                v = (j > 0 && j < ny - 1) ? (in[idx(i, j - 1, k)] & in[idx(i, j + 1, k)]) : 0;
+              } else {
+                int kd = std::min(k + 1, nz - 1);  // clamp below the grid
+                v = (k > 0) ? (in[idx(i, j, k - 1)] & in[idx(i, j, kd)]) : 0;
+              }
+            }
+            out[idx(i, j, k)] = v;
+          }
+    });
+  }
+  for (auto &th : pool) th.join();
+}
+}  // namespace
+
+void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius) {
+  const size_t nv = (size_t)nx * ny * nz;
+  auto idx = [&](int i, int j, int k) { return ((size_t)k * ny + j) * nx + i; };
+  std::vector<uint8_t> cur(nv), tmp(nv), tmp2(nv);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        uint8_t ok = (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0) ? 1 : 0;
+        const uint8_t t = rows[idx(i, j, k)];
+        for (int dk = -1; dk <= 1 && ok; ++dk)
+          for (int dj = -1; dj <= 1 && ok; ++dj)
+            for (int di = -1; di <= 1 && ok; ++di) {
+              int kk = std::min(k + dk, nz - 1);
+              if (rows[idx(i + di, j + dj, kk)] != t) ok = 0;
+            }
+        cur[idx(i, j, k)] = ok;
+        radius[idx(i, j, k)] = ok;
+      }
+  for (int r = 2; r <= kSafeMax; ++r) {
+    erode_axis(cur.data(), tmp.data(), nx, ny, nz, 0);
+    erode_axis(tmp.data(), tmp2.data(), nx, ny, nz, 1);
+    erode_axis(tmp2.data(), cur.data(), nx, ny, nz, 2);
+    bool any = false;
+    for (size_t v = 0; v < nv; ++v) {
+      radius[v] += cur[v];
+      any |= cur[v] != 0;
+    }
+    if (!any) break;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // Binary formats, src/mc3d_io.cu.
 
 namespace {

@@ -19,7 +19,12 @@
 //  - the next voxel index comes from the face test of the DDA instead of three floor divisions;
 //  - the HG, roulette and step-length math runs only for lanes that interact, behind a
 //    wave-uniform branch; lanes that only cross a voxel face still advance the stream by the
-//    four draws the reference consumes on every step.
+//    four draws the reference consumes on every step;
+//  - plain voxel crossings inside a region of one tissue run in a tight inner loop with no
+//    memory access: each voxel word carries a safe radius (Chebyshev distance within which all
+//    voxels share its tissue and no mirror wall or top surface is reached), and a packet that
+//    stays inside the safe cube of its anchor voxel crosses faces without loading voxels or
+//    testing for Fresnel.  Such a crossing is exactly the general step's crossing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,8 +94,8 @@ struct RowConst {
 };
 
 struct KArgs {
-  // geometry
-  const uint8_t *vox;
+  // geometry: voxel words, tissue row in bits 0-7, safe radius in bits 8-15
+  const uint16_t *vox;
   int nx, ny, nz;
   float dx, dy, dz, Lx, Ly;
   // optical properties
@@ -124,6 +129,8 @@ struct KArgs {
   unsigned long long n_chains;
   const volatile int *cancel;
   int poll_interval;
+  int fast_min;  // inner fast-crossing loop continues while >= fast_min lanes can cross
+  int fast_max;  // and at most fast_max iterations
   ymc_photon_record *records;
   unsigned long long n_records;
 };
@@ -141,6 +148,7 @@ struct Packet {
   float iux, iuy, iuz;
   int i, j, k, t;
   uint32_t nstep;  // steps taken by this packet
+  int ai, aj, ak, ar;  // anchor voxel and its safe radius (ar < 0: no safe cube)
 };
 
 // Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
@@ -150,12 +158,17 @@ constexpr uint32_t kLaunchSteps = 45;
 // Step cap per packet (a packet that reaches it is killed; see the kill rule in step_packet).
 constexpr uint32_t kMaxSteps = 1u << 24;
 
-__device__ __forceinline__ int voxel_row(const KArgs &a, int i, int j, int k) {
+__device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int k) {
   int ic = min(max(i, 0), a.nx - 1);
   int jc = min(max(j, 0), a.ny - 1);
   int kc = min(max(k, 0), a.nz - 1);
   return a.vox[((size_t)kc * a.ny + jc) * a.nx + ic];
 }
+
+struct LaneAcc {
+  uint32_t steps, fresnel, photons;
+  uint64_t absorbed;
+};
 
 template <bool UNIFORM>
 __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float w0) {
@@ -178,14 +191,57 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
   p.i = grid_coord(p.x, 0.0f, a.dx);
   p.j = grid_coord(p.y, 0.0f, a.dy);
   p.k = 0;                                         // grid_coord(0, +1) = 0
-  p.t = voxel_row(a, p.i, p.j, 0);                 // uz > 0: never the air override
+  const uint32_t w = voxel_word(a, p.i, p.j, 0);   // uz > 0: never the air override
+  p.t = (int)(w & 0xffu);
   p.nstep = 0;
+  p.ai = p.i;
+  p.aj = p.j;
+  p.ak = 0;
+  p.ar = (int)(w >> 8);
 }
 
-struct LaneAcc {
-  uint32_t steps, fresnel, photons;
-  uint64_t absorbed;
-};
+// A plain crossing of a voxel face inside the anchor's safe cube: hit, positive step length,
+// destination inside the cube (same tissue, so ni == nt and no Fresnel draw), new position
+// strictly between the mirror walls, live packet in tissue.  For such a step the general step
+// below reduces to: move to the face, advance the indices, s -= db*mu_t, four draws.
+// Returns true and commits when the packet's next step is such a crossing.
+template <bool COUNTERS>
+__device__ __forceinline__ bool fast_crossing(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float mu_t,
+                                              LaneAcc &acc) {
+  const float bx = (float)(p.ux >= 0.0f ? p.i + 1 : p.i) * a.dx;
+  const float by = (float)(p.uy >= 0.0f ? p.j + 1 : p.j) * a.dy;
+  const float bz = (float)(p.uz >= 0.0f ? p.k + 1 : p.k) * a.dz;
+  const float tx = (bx - p.x) * p.iux;
+  const float ty = (by - p.y) * p.iuy;
+  const float tz = (bz - p.z) * p.iuz;
+  const float db = fminf(tx, fminf(ty, tz));
+  const float dbm = __fmul_rn(db, mu_t);  // rounded product, as in step_packet
+  const float nx = (db == tx) ? bx : fmaf(p.ux, db, p.x);
+  const float ny = (db == ty) ? by : fmaf(p.uy, db, p.y);
+  const float nz = (db == tz) ? bz : fmaf(p.uz, db, p.z);
+  const int ni = p.i + (p.ux >= 0.0f ? (nx >= bx ? 1 : 0) : (nx <= bx ? -1 : 0));
+  const int nj = p.j + (p.uy >= 0.0f ? (ny >= by ? 1 : 0) : (ny <= by ? -1 : 0));
+  const int nk = p.k + (p.uz >= 0.0f ? (nz >= bz ? 1 : 0) : (nz <= bz ? -1 : 0));
+  const bool ok = dbm <= p.s && db > 0.0f && mu_t > 0.0f && abs(ni - p.ai) <= p.ar && abs(nj - p.aj) <= p.ar &&
+                  abs(nk - p.ak) <= p.ar && nx > 0.0f && nx < a.Lx && ny > 0.0f && ny < a.Ly &&
+                  p.nstep + 1 < kMaxSteps;
+  if (ok) {
+    p.x = nx;
+    p.y = ny;
+    p.z = nz;
+    p.i = ni;
+    p.j = nj;
+    p.k = nk;
+    p.s -= dbm;
+    mwc_next(rx, ra);
+    mwc_next(rx, ra);
+    mwc_next(rx, ra);
+    mwc_next(rx, ra);
+    p.nstep++;
+    if (COUNTERS) acc.steps++;
+  }
+  return ok;
+}
 
 // Outcome of one step.
 enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
@@ -208,7 +264,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const float tz = (bz - p.z) * p.iuz;
   const float db = fminf(tx, fminf(ty, tz));
 
-  const float dbm = db * mu_t;
+  const float dbm = __fmul_rn(db, mu_t);  // rounded product (no contraction into s - db*mu_t)
   const bool hit = dbm <= p.s;                      // :197
   const float ds = hit ? dbm : p.s;                 // :198
   const float hop = hit ? db : ds * c1.z;           // distance moved
@@ -221,8 +277,9 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const int ni_ = p.i + (p.ux >= 0.0f ? (nx >= bx ? 1 : 0) : (nx <= bx ? -1 : 0));
   const int nj_ = p.j + (p.uy >= 0.0f ? (ny >= by ? 1 : 0) : (ny <= by ? -1 : 0));
   const int nk_ = p.k + (p.uz >= 0.0f ? (nz >= bz ? 1 : 0) : (nz <= bz ? -1 : 0));
-  int next_t = voxel_row(a, ni_, nj_, nk_);
-  next_t = (nz <= 0.0f && p.uz < 0.0f) ? YMC_TISSUE_AIR : next_t;
+  const uint32_t nw = voxel_word(a, ni_, nj_, nk_);
+  const bool air_override = nz <= 0.0f && p.uz < 0.0f;
+  const int next_t = air_override ? YMC_TISSUE_AIR : (int)(nw & 0xffu);
   const float nt = tab[next_t].r0.w;
 
   float nux = p.ux, nuy = p.uy, nuz = p.uz;
@@ -233,13 +290,13 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     const bool xi = p.i != ni_, yj = p.j != nj_, zk = p.k != nk_;
     float cos_ai = xi ? p.ux : (yj ? p.uy : p.uz);
     cos_ai = fminf(fabsf(cos_ai), 1.0f);
-    const float ni_nt = ni / nt;
-    const float sin_ai = sqrtf(fmaxf(1.0f - cos_ai * cos_ai, 0.0f));  // sin(acos(c))
+    const float ni_nt = ni * rcp(nt);
+    const float sin_ai = sqrtf((1.0f - cos_ai) * (1.0f + cos_ai));  // sin(acos(c)), well conditioned
     const float sin_at = ni_nt * sin_ai;
     const float cos_at = sqrtf(1.0f - sin_at * sin_at);
-    const float R1 = (sin_ai * cos_at - cos_ai * sin_at) / (sin_ai * cos_at + cos_ai * sin_at);
+    const float R1 = (sin_ai * cos_at - cos_ai * sin_at) * rcp(sin_ai * cos_at + cos_ai * sin_at);
     const float r2d = a.fresnel_exact ? (cos_ai * cos_at + sin_ai * sin_at) : (cos_ai * cos_at + sin_at * sin_at);
-    const float R2 = (cos_ai * cos_at - sin_ai * sin_at) / r2d;
+    const float R2 = (cos_ai * cos_at - sin_ai * sin_at) * rcp(r2d);
     float R = 0.5f * R1 * R1 * (1.0f + R2 * R2);
     R = (sin_at > 1.0f) ? 1.0f : R;
     const float eps = u32_co(mwc_next(rx, ra));
@@ -341,6 +398,11 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.j = nj_;
     p.k = nk_;
     p.t = next_t;
+    // the committed voxel is the one just loaded: it becomes the anchor
+    p.ai = ni_;
+    p.aj = nj_;
+    p.ak = nk_;
+    p.ar = air_override ? -1 : (int)(nw >> 8);
   }
   if (COUNTERS) acc.steps++;
 
@@ -405,11 +467,24 @@ __device__ __forceinline__ void tally_packet(const Packet &p, const KArgs &a, ui
 
 constexpr int kBlock = 256;
 
+// Byte offset of the per-table totals in the kernel's dynamic LDS (8-byte aligned).
+__host__ __device__ inline size_t lds_tot_offset(int tables, int types) {
+  size_t off = (size_t)tables * types * sizeof(RowConst) + (size_t)tables * sizeof(float);
+  return (off + 7) & ~(size_t)7;
+}
+__host__ __device__ inline size_t lds_bytes(int tables, int types) {
+  return lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
+}
+
 template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD>
 __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
+  // LDS: optical-property rows, launch weights, then the workgroup's per-table totals, which
+  // are accumulated with LDS atomics and added to the device totals once, when the block exits.
   RowConst *s_rows = reinterpret_cast<RowConst *>(lds_raw);
   float *s_w0 = reinterpret_cast<float *>(s_rows + a.num_tables * a.num_types);
+  unsigned long long *s_tot = reinterpret_cast<unsigned long long *>(lds_raw + lds_tot_offset(a.num_tables, a.num_types));
+  for (int q = threadIdx.x; q < a.num_tables * kTotFields; q += blockDim.x) s_tot[q] = 0;
   {
     const int nrow = a.num_tables * a.num_types;
     const float4 *src = reinterpret_cast<const float4 *>(a.rows);
@@ -442,14 +517,14 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       if (left == 0 && !drained) {
         // Chain finished: flush its counters, then take the next chain.  One atomic per wave.
         if (COUNTERS && acc.photons) {
-          unsigned long long *tt = a.tot + (size_t)table * kTotFields;
+          unsigned long long *tt = s_tot + (size_t)table * kTotFields;
           atomicAdd(&tt[kPhotons], (unsigned long long)acc.photons);
           atomicAdd(&tt[kSteps], (unsigned long long)acc.steps);
           atomicAdd(&tt[kFresnel], (unsigned long long)acc.fresnel);
           atomicAdd(&tt[kAw], (unsigned long long)acc.absorbed);
           acc = LaneAcc{0, 0, 0, 0};
         } else if (!COUNTERS && acc.photons) {
-          atomicAdd(&a.tot[(size_t)table * kTotFields + kPhotons], (unsigned long long)acc.photons);
+          atomicAdd(&s_tot[(size_t)table * kTotFields + kPhotons], (unsigned long long)acc.photons);
           acc.photons = 0;
         }
         const uint64_t want = __ballot(1);
@@ -477,9 +552,18 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       }
     }
     if (__ballot(alive) == 0) break;
+    // Plain crossings inside safe cubes, while enough lanes of the wave have one.
+    if (a.fast_min <= 64) {
+      const float mu_t = alive ? s_rows[(size_t)table * a.num_types + p.t].r1.x : 0.0f;
+      for (int it = 0; it < a.fast_max; ++it) {
+        bool f = false;
+        if (alive) f = fast_crossing<COUNTERS>(p, rx, ra, a, mu_t, acc);
+        if (__popcll(__ballot(f)) < (unsigned)a.fast_min) break;
+      }
+    }
     if (alive) {
       const RowConst *tab = s_rows + (size_t)table * a.num_types;
-      unsigned long long *tt = a.tot + (size_t)table * kTotFields;
+      unsigned long long *tt = s_tot + (size_t)table * kTotFields;
       const int res = step_packet<COUNTERS>(p, rx, ra, a, tab, tt, acc);
       if (res != kLive) {
         tally_packet<RADIAL>(p, a, table, tt);
@@ -503,15 +587,20 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       }
     }
   }
-  // Final flush.
+  // Final flush: lane counters into LDS, then the block's totals into the device totals.
   if (acc.photons) {
-    unsigned long long *tt = a.tot + (size_t)table * kTotFields;
+    unsigned long long *tt = s_tot + (size_t)table * kTotFields;
     atomicAdd(&tt[kPhotons], (unsigned long long)acc.photons);
     if (COUNTERS) {
       atomicAdd(&tt[kSteps], (unsigned long long)acc.steps);
       atomicAdd(&tt[kFresnel], (unsigned long long)acc.fresnel);
       atomicAdd(&tt[kAw], (unsigned long long)acc.absorbed);
     }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < a.num_tables * kTotFields; q += blockDim.x) {
+    const unsigned long long v = s_tot[q];
+    if (v) atomicAdd(&a.tot[q], v);
   }
 }
 
@@ -700,14 +789,24 @@ void ymc_engine_destroy(ymc_engine *e) {
   delete e;  // DevBuf destructors free device memory
 }
 
+ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *out) {
+  if (!rows || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "ymc_safe_radius: bad arguments");
+  safe_radius(rows, nx, ny, nz, out);
+  return YMC_OK;
+}
+
 ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
   ymc_status st = check_engine(e);
   if (st) return st;
   if (!g || !g->tissue || g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || !(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0))
     return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: invalid geometry");
   size_t nv = (size_t)g->nx * g->ny * g->nz;
-  if ((st = e->vox.ensure(nv))) return st;
-  HIP_TRY(hipMemcpy(e->vox.p, g->tissue, nv, hipMemcpyHostToDevice));
+  std::vector<uint8_t> radius(nv);
+  safe_radius(g->tissue, g->nx, g->ny, g->nz, radius.data());
+  std::vector<uint16_t> words(nv);
+  for (size_t v = 0; v < nv; ++v) words[v] = (uint16_t)(g->tissue[v] | (radius[v] << 8));
+  if ((st = e->vox.ensure(nv * 2))) return st;
+  HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 2, hipMemcpyHostToDevice));
   e->nx = g->nx;
   e->ny = g->ny;
   e->nz = g->nz;
@@ -859,7 +958,7 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   if (e->n_records) HIP_TRY(hipMemsetAsync(e->records.p, 0, e->n_records * sizeof(ymc_photon_record), s));
 
   KArgs a{};
-  a.vox = e->vox.as<uint8_t>();
+  a.vox = e->vox.as<uint16_t>();
   a.nx = e->nx;
   a.ny = e->ny;
   a.nz = e->nz;
@@ -904,6 +1003,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   *e->cancel_host = (prm.cancel && *prm.cancel) ? 1 : 0;
   a.cancel = e->cancel_dev;
   a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
+  a.fast_min = prm.fast_min_lanes > 0 ? prm.fast_min_lanes : 8;
+  a.fast_max = prm.fast_max_iters > 0 ? prm.fast_max_iters : 64;
   a.records = e->records.as<ymc_photon_record>();
   a.n_records = e->n_records;
 
@@ -911,7 +1012,7 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   const bool counters = (prm.flags & YMC_FLAG_COUNTERS) != 0;
   const bool record = (prm.flags & YMC_FLAG_RECORD) != 0;
   const void *kern = select_kernel(uniform, counters, e->nr > 0, record);
-  const size_t lds = (size_t)e->num_tables * e->num_types * sizeof(RowConst) + e->num_tables * sizeof(float);
+  const size_t lds = lds_bytes(e->num_tables, e->num_types);
   if (lds > 64 * 1024) return fail(YMC_ERR_ARG, "optical-property tables need %zu B of LDS (max 65536)", lds);
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds));
