@@ -32,6 +32,7 @@ PEAK_FP32_TFLOPS = 157.3
 WORKLOADS = {
     "skin_vessels": ("skin_vessels", 100_000_000),
     "skin_slab_sphere": ("skin_slab_sphere", 1_000_000),
+    "batched_spectra": ("batched_spectra", 0),
 }
 
 
@@ -59,9 +60,16 @@ def main() -> int:
     from ymc3d_amd import configs
 
     fn, default_n = WORKLOADS[args.workload]
-    n_photons = args.photons or default_n
-    scene = getattr(configs, fn)(photons=n_photons)
+    if fn == "batched_spectra":
+        scene = configs.batched_spectra()
+        n_photons = int(np.sum(scene.params["budgets"]))
+    else:
+        n_photons = args.photons or default_n
+        scene = getattr(configs, fn)(photons=n_photons)
     params = dict(scene.params)
+    from ymc3d_amd import autotune
+
+    params.update(autotune.tuned(scene, scene.params))
     if args.pool_lanes:
         params["pool_lanes"] = args.pool_lanes
     if args.chain_lanes:
@@ -74,8 +82,9 @@ def main() -> int:
     eng.prepare(**params, flags=y.FLAG_COUNTERS)
     eng.run()
     counted = eng.read_tally()
-    steps = int(counted.totals[0]["steps"])
-    fresnel = int(counted.totals[0]["fresnel"])
+    steps = int(counted.totals["steps"].sum())
+    fresnel = int(counted.totals["fresnel"].sum())
+    tables = len(counted.totals)
 
     eng.prepare(**params)
     for _ in range(args.warmup):
@@ -88,7 +97,7 @@ def main() -> int:
         res = eng.read_tally()
         kernel_ms.append(res.kernel_ms)
     elapsed = time.perf_counter() - t0
-    assert int(res.totals[0]["photons"]) == n_photons
+    assert int(res.totals["photons"].sum()) == n_photons
     assert res.totals.tobytes() != b"" and np.array_equal(res.R_w, res.R_w)
 
     value = args.steps * n_photons / elapsed
@@ -115,6 +124,10 @@ def main() -> int:
             "voxel_cm": scene.voxel[0],
             "tally": [params.get("tally_nx", 0) or scene.shape[0], params.get("tally_ny", 0) or scene.shape[1]],
             "pool_lanes": eng.pool_lanes(),
+            "tables": tables,
+            "spectra_per_s": args.steps / elapsed if tables > 1 else None,
+            "launch_params": {k: params[k] for k in ("pool_lanes", "fast_min_lanes", "fast_max_iters",
+                                                      "compact_interval") if k in params},
             "steps_per_photon": steps / n_photons,
             "fresnel_per_photon": fresnel / n_photons,
             "kernel_ms": kms,
@@ -141,14 +154,26 @@ def cpu_baseline(scene, tab, params, n_photons, seconds):
     """CPU oracle (all host threads) on a bounded sample of the same workload."""
     from oracle import oracle as O
 
+    import numpy as np
+
     threads = O.lib().ymc_oracle_max_threads()
+
+    def run(count):
+        if "budgets" in scene.params:  # batched: every table scaled to the sample size
+            b = np.asarray(scene.params["budgets"], dtype=np.float64)
+            budgets = np.maximum(1, np.round(b * count / b.sum())).astype(np.uint64)
+            O.run_scene(scene, tab, budgets=budgets)
+            return int(budgets.sum())
+        O.run_scene(scene, tab, num_photons=count)
+        return count
+
     probe = 20_000
     t0 = time.perf_counter()
-    O.run_scene(scene, tab, num_photons=probe)
+    probe = run(probe)
     rate = probe / (time.perf_counter() - t0)
     sample = int(min(n_photons, max(probe, rate * seconds)))
     t0 = time.perf_counter()
-    O.run_scene(scene, tab, num_photons=sample)
+    sample = run(sample)
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "photons/s", "cores": threads, "kind": "port",
             "sample": f"{sample} packets of {scene.name} (same geometry, tables, schedule rule), {dt:.1f} s"}

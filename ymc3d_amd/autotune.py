@@ -1,0 +1,117 @@
+"""Autotuner for the persistent transport launch.
+
+Searches the launch parameters that do not change results (tallies are bit-identical for any
+value of them, which the GPU tests check):
+  pool_lanes        resident lanes (workgroups x 256), up to full occupancy
+  fast_min_lanes    the safe-cube crossing loop continues while this many lanes of a wave can cross
+  fast_max_iters    cap on consecutive fast iterations
+  compact_interval  steps between the cancel-flag polls
+on a sample budget of the actual workload, then stores the best setting in a JSON file keyed by
+device name and workload signature.  Engine runs pick it up through `tuned(...)`.
+"""
+from __future__ import annotations
+
+import hashlib
+import itertools
+import json
+import os
+import time
+
+import numpy as np
+
+CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                                          "autotune.json"))
+SEARCH = {
+    "pool_frac": (1.0, 0.75, 0.5),
+    "fast_min_lanes": (4, 8, 16, 32),
+    "fast_max_iters": (16, 64),
+    "compact_interval": (1024,),
+}
+
+
+def signature(scene, params: dict) -> str:
+    """Workload key: geometry shape and voxel, table count, illumination and tally layout."""
+    h = hashlib.sha1()
+    h.update(np.ascontiguousarray(scene.file_values).tobytes())
+    h.update(np.ascontiguousarray(scene.records, dtype=np.float64).tobytes())
+    for k in ("illumination", "tally_nx", "tally_ny", "radial_bins"):
+        h.update(f"{k}={params.get(k, 0)};".encode())
+    return h.hexdigest()[:16]
+
+
+def device_name() -> str:
+    try:
+        import subprocess
+
+        out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            if "Marketing Name" in line and "AMD" in line and "Instinct" in line:
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "gpu"
+
+
+def _load() -> dict:
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            return json.load(f)
+    return {}
+
+
+def tuned(scene, params: dict) -> dict:
+    """Launch parameters stored for this workload (empty if never tuned)."""
+    return dict(_load().get(device_name(), {}).get(signature(scene, params), {}).get("params", {}))
+
+
+def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, verbose: bool = True) -> dict:
+    """Times every combination of SEARCH on a sample budget; stores and returns the best."""
+    params = dict(scene.params)
+    if sample_photons:
+        if "budgets" in params:
+            b = np.asarray(params["budgets"], dtype=np.float64)
+            params["budgets"] = np.maximum(1, b * sample_photons / b.sum()).astype(np.uint64)
+        else:
+            params["num_photons"] = sample_photons
+    scene.load(engine)
+    engine.prepare(**params)
+    engine.run()
+    full = engine.pool_lanes()
+    results = []
+    for pf, fm, fx, ci in itertools.product(*SEARCH.values()):
+        cand = {"pool_lanes": max(256, int(full * pf) // 256 * 256), "fast_min_lanes": fm, "fast_max_iters": fx,
+                "compact_interval": ci}
+        engine.prepare(**params, **cand)
+        ms = min(engine.run() for _ in range(repeats))
+        results.append((ms, cand))
+        if verbose:
+            print(json.dumps({"kernel_ms": round(ms, 3), **cand}), flush=True)
+    results.sort(key=lambda r: r[0])
+    best_ms, best = results[0]
+    db = _load()
+    entry = db.setdefault(device_name(), {})
+    entry[signature(scene, scene.params)] = {"params": best, "kernel_ms": best_ms, "scene": scene.name,
+                                             "sample_photons": sample_photons, "tuned_at": time.time(),
+                                             "worst_ms": results[-1][0]}
+    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+    with open(CACHE + ".tmp", "w") as f:
+        json.dump(db, f, indent=1)
+    os.replace(CACHE + ".tmp", CACHE)
+    return best
+
+
+if __name__ == "__main__":
+    import argparse
+    import sys
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import ymc3d_amd as y
+    from ymc3d_amd import configs
+
+    ap = argparse.ArgumentParser(description="tune launch parameters for a workload")
+    ap.add_argument("workload", choices=["skin_vessels", "skin_slab_sphere", "batched_spectra"])
+    ap.add_argument("--sample", type=int, default=20_000_000)
+    a = ap.parse_args()
+    eng = y.Engine(0)
+    sc = getattr(configs, a.workload)()
+    print("best:", json.dumps(tune(eng, sc, a.sample)))
