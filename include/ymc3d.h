@@ -69,6 +69,22 @@ typedef struct ymc_geometry {
   const uint8_t *tissue;  /* nz*ny*nx rows, x fastest */
 } ymc_geometry;
 
+/* Parametric geometry evaluated per voxel on the device: a top layer over a semi-infinite bottom
+ * layer with a cylinder parallel to y.  Voxel (i,j,k) has its centre at (i+0.5, j+0.5, k+0.5) in
+ * voxel units; it belongs to the cylinder when (i+0.5-vessel_x)^2 + (k+0.5-vessel_z)^2 <=
+ * vessel_r^2 (binary32, each operation rounded), else to the top layer when k < top_voxels, else to
+ * the bottom layer.  Used for batched runs in which every table has its own geometry (the voxel
+ * grid these parameters describe is what ymc_layered_voxelize writes).  No counterpart in the
+ * reference, which reads voxel grids only (src/mc3d_io.cu:107-169). */
+typedef struct ymc_layered {
+  int32_t top_voxels;            /* thickness of the top layer in voxels */
+  int32_t row_top, row_bottom;   /* optical-property rows of the layers */
+  int32_t row_vessel;            /* row of the cylinder, or -1 for none */
+  float vessel_x, vessel_z;      /* cylinder axis, voxel units */
+  float vessel_r;                /* cylinder radius, voxel units */
+  float reserved;
+} ymc_layered;
+
 /* Optical properties.  Mirrors opticalprops_t (src/mc3d_types.h:28-36): num_types rows
  * including row 0 = ambient (mua=mus=g=0, n=1).  num_tables > 1 stacks independent
  * tables (wavelengths / property sets) for a batched run. */
@@ -187,6 +203,9 @@ ymc_status ymc_write_property(const char *path, int32_t dims, int32_t nx, int32_
                               double dx, double dy, double dz, const double *data);
 void ymc_free(void *p);
 
+/* Writes the rows of a parametric geometry (see ymc_layered) as a voxel grid [nz][ny][nx]. */
+ymc_status ymc_layered_voxelize(const ymc_layered *g, int32_t nx, int32_t ny, int32_t nz, uint8_t *rows);
+
 /* Safe radius per voxel (host): the largest r <= 31 such that every voxel within Chebyshev
  * distance r has the same row, lies inside the lateral grid and not above the top layer (below
  * the grid the bottom layer repeats).  The engine packs it into its voxel words; exported as a
@@ -205,6 +224,10 @@ void ymc_engine_destroy(ymc_engine *e);
 
 /* Uploads the voxel grid (the reference's 3-D texture, src/mc3d_gpu.cu:398-421). */
 ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g);
+/* Selects a parametric geometry instead of a voxel grid: grid size nx*ny*nz of voxels dx*dy*dz,
+ * geometry per table (num_tables entries, matching the optical-property tables at run time). */
+ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t nz, float dx, float dy, float dz,
+                                  const ymc_layered *per_table, int32_t num_tables);
 /* Uploads the optical-property tables (opticalprops_transfer_to_device, src/mc3d_io.cu:70-86). */
 ymc_status ymc_engine_set_optprops(ymc_engine *e, const ymc_optprops *p);
 
@@ -246,6 +269,31 @@ ymc_status ymc_run_3dmc_gpu(int32_t nx, int32_t ny, int32_t nz, float dx, float 
                             uint64_t seed, const uint32_t *table, uint32_t table_len, double *R,
                             double *R_tot, double *T, double *T_tot, float *A,
                             int32_t *num_finished);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Reflectance surrogate: MLP with GELU hidden layers trained and served on the device with BF16
+ * matrix-core products (FP32 accumulation) and Adam on FP32 master weights.  No counterpart in the
+ * reference.  Parameters are one flat FP32 vector: for each layer l, W_l [in_l x out_l] row-major
+ * followed by b_l [out_l]. */
+
+typedef struct ymc_mlp ymc_mlp;
+
+/* widths: n_widths layer widths, input first (e.g. 12, 256, 256, 256, 33). He initialisation from seed. */
+ymc_status ymc_mlp_create(int32_t device, const int32_t *widths, int32_t n_widths, uint64_t seed, ymc_mlp **out);
+void ymc_mlp_destroy(ymc_mlp *n);
+uint64_t ymc_mlp_num_params(const ymc_mlp *n);
+ymc_status ymc_mlp_set_params(ymc_mlp *n, const float *params, uint64_t count);  /* resets Adam state */
+ymc_status ymc_mlp_get_params(ymc_mlp *n, float *params, uint64_t count);
+/* Outputs y [rows x out] for inputs x [rows x in], in launches of 4096 rows. */
+ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y);
+/* Mean-squared-error loss and its gradient (flat, like the parameters) for one batch (test hook). */
+ymc_status ymc_mlp_gradients(ymc_mlp *n, const float *x, const float *y, int32_t rows, float *grad, float *loss);
+/* Uploads a training set (x [rows x in], y [rows x out]). */
+ymc_status ymc_mlp_set_data(ymc_mlp *n, const float *x, const float *y, int64_t rows);
+/* Adam epochs over the training set in shuffled minibatches, learning rate cosine-decayed from lr to
+ * lr_final; epoch_loss (optional, [epochs]) receives the last minibatch loss of each epoch. */
+ymc_status ymc_mlp_train(ymc_mlp *n, int32_t epochs, int32_t batch, float lr, float lr_final, uint64_t seed,
+                         float *epoch_loss);
 
 #ifdef __cplusplus
 }

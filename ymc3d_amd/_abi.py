@@ -25,6 +25,17 @@ class Geometry(C.Structure):
                 ("tissue", C.POINTER(C.c_uint8))]
 
 
+class Layered(C.Structure):
+    _fields_ = [("top_voxels", C.c_int32), ("row_top", C.c_int32), ("row_bottom", C.c_int32),
+                ("row_vessel", C.c_int32), ("vessel_x", C.c_float), ("vessel_z", C.c_float),
+                ("vessel_r", C.c_float), ("reserved", C.c_float)]
+
+
+LAYERED_DTYPE = np.dtype([("top_voxels", np.int32), ("row_top", np.int32), ("row_bottom", np.int32),
+                          ("row_vessel", np.int32), ("vessel_x", np.float32), ("vessel_z", np.float32),
+                          ("vessel_r", np.float32), ("reserved", np.float32)])
+
+
 class OptProps(C.Structure):
     _fields_ = [("num_types", C.c_int32), ("num_tables", C.c_int32),
                 ("mua", C.POINTER(C.c_float)), ("mus", C.POINTER(C.c_float)),
@@ -113,6 +124,9 @@ _SIGS = {
                                      C.c_double, C.c_double, P(C.c_double)]),
     "ymc_free": (None, [C.c_void_p]),
     "ymc_safe_radius": (C.c_int, [P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8)]),
+    "ymc_layered_voxelize": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8)]),
+    "ymc_engine_set_layered": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                         C.c_float, C.c_void_p, C.c_int32]),
     "ymc_engine_create": (C.c_int, [C.c_int32, P(C.c_uint32), C.c_uint32, P(C.c_void_p)]),
     "ymc_engine_destroy": (None, [C.c_void_p]),
     "ymc_engine_set_geometry": (C.c_int, [C.c_void_p, P(Geometry)]),
@@ -132,6 +146,15 @@ _SIGS = {
                                    P(C.c_float), C.c_int32, C.c_uint64, P(C.c_uint32), C.c_uint32,
                                    P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_float),
                                    P(C.c_int32)]),
+    "ymc_mlp_create": (C.c_int, [C.c_int32, P(C.c_int32), C.c_int32, C.c_uint64, P(C.c_void_p)]),
+    "ymc_mlp_destroy": (None, [C.c_void_p]),
+    "ymc_mlp_num_params": (C.c_uint64, [C.c_void_p]),
+    "ymc_mlp_set_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_uint64]),
+    "ymc_mlp_get_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_uint64]),
+    "ymc_mlp_forward": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float)]),
+    "ymc_mlp_gradients": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float), P(C.c_float)]),
+    "ymc_mlp_set_data": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64]),
+    "ymc_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, P(C.c_float)]),
 }
 
 # Entry points of the product library, in the order include/ymc3d.h declares them.

@@ -146,3 +146,85 @@ def van_de_hulst_slab(photons: int = 1_000_000, seed: int = 4) -> Scene:
     rec = np.array([[[10.0, 90.0, 0.75, 1.0]]])
     params = dict(num_photons=photons, seed=seed, illumination=0)
     return Scene("van_de_hulst_slab", vals, (0.01, 0.01, d), rec, params)
+
+
+# ------------------------------------------------------------------------------------------------
+# Two-layer skin with one vessel, 128x128x64 voxels of 0.002 cm, one parametric geometry per
+# table: the training workload of the reflectance surrogate.  Twelve parameters per sample:
+#   0-2  mua of epidermis, dermis, blood  [1/cm]   logU(0.1, 50), logU(0.1, 50), logU(1, 300)
+#   3-5  mus' of epidermis, dermis, blood [1/cm]   logU(5, 50)
+#   6-8  g of epidermis, dermis, blood             U(0.7, 0.95)
+#   9    epidermis thickness [cm]                  2..8 voxels (40-160 um), uniform
+#   10   vessel axis depth [cm]                    U(0.02, 0.1)
+#   11   vessel diameter [cm]                      U(0.003, 0.02)
+# mus = mus'/(1-g) (prepare_optprops.m:35); n = 1.44 (epidermis), 1.39 (dermis, blood).  Pencil beam
+# at the surface centre; targets are 32 radial reflectance bins of 0.004 cm plus total reflectance.
+
+SKIN2_SHAPE = (128, 128, 64)
+SKIN2_VOXEL = 0.002
+SKIN2_RADIAL = (32, 0.004)
+PARAM_NAMES = ("mua_epi", "mua_der", "mua_blood", "musp_epi", "musp_der", "musp_blood", "g_epi", "g_der",
+               "g_blood", "epi_thickness", "vessel_depth", "vessel_diameter")
+PARAM_LOG = np.array([1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0], dtype=bool)
+PARAM_LO = np.array([0.1, 0.1, 1.0, 5.0, 5.0, 5.0, 0.7, 0.7, 0.7, 2 * SKIN2_VOXEL, 0.02, 0.003])
+PARAM_HI = np.array([50.0, 50.0, 300.0, 50.0, 50.0, 50.0, 0.95, 0.95, 0.95, 8 * SKIN2_VOXEL, 0.10, 0.020])
+
+
+def sample_skin2_params(n: int, seed: int, outlier_frac: float = 0.0) -> np.ndarray:
+    """Seeded parameter vectors [n, 12]; outlier_frac of them drawn from ranges widened by a quarter of
+    their (log) width on both sides, so some parameters fall outside the training ranges."""
+    rng = np.random.default_rng(seed)
+    lo, hi = PARAM_LO.copy(), PARAM_HI.copy()
+    u = rng.uniform(size=(n, 12))
+    if outlier_frac > 0:
+        out = rng.uniform(size=n) < outlier_frac
+        u[out] = rng.uniform(-0.25, 1.25, size=(out.sum(), 12))
+    P = np.where(PARAM_LOG, np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo))), lo + u * (hi - lo))
+    P[:, 9] = np.clip(np.rint(P[:, 9] / SKIN2_VOXEL), 1, 20) * SKIN2_VOXEL  # whole voxels
+    P[:, 6:9] = np.clip(P[:, 6:9], 0.5, 0.98)
+    P[:, 10:12] = np.maximum(P[:, 10:12], SKIN2_VOXEL)
+    return P
+
+
+def skin2_tables(P: np.ndarray):
+    """Optical-property tables [tables, 4 rows] (row 0 ambient) and layered geometries for parameter rows."""
+    import ymc3d_amd._abi as abi
+
+    P = np.atleast_2d(P)
+    L = P.shape[0]
+    mua = np.zeros((L, 4), np.float32)
+    mus = np.zeros((L, 4), np.float32)
+    g = np.zeros((L, 4), np.float32)
+    n = np.ones((L, 4), np.float32)
+    for r in range(3):
+        mua[:, r + 1] = P[:, r]
+        g[:, r + 1] = P[:, 6 + r]
+        mus[:, r + 1] = P[:, 3 + r] / (1.0 - P[:, 6 + r])
+    n[:, 1], n[:, 2], n[:, 3] = 1.44, 1.39, 1.39
+    geo = np.zeros(L, dtype=abi.LAYERED_DTYPE)
+    nx = SKIN2_SHAPE[0]
+    geo["top_voxels"] = np.rint(P[:, 9] / SKIN2_VOXEL).astype(np.int32)
+    geo["row_top"], geo["row_bottom"], geo["row_vessel"] = 1, 2, 3
+    geo["vessel_x"] = nx / 2.0
+    geo["vessel_z"] = P[:, 10] / SKIN2_VOXEL
+    geo["vessel_r"] = P[:, 11] / 2.0 / SKIN2_VOXEL
+    return (mua, mus, g, n), geo
+
+
+def skin2_params(photons: int) -> dict:
+    nx, ny, _ = SKIN2_SHAPE
+    return dict(num_photons=photons, illumination=1, pencil_x=nx * SKIN2_VOXEL / 2, pencil_y=ny * SKIN2_VOXEL / 2,
+                radial_bins=SKIN2_RADIAL[0], radial_dr=SKIN2_RADIAL[1], tally_nx=1, tally_ny=1,
+                tally_dx=nx * SKIN2_VOXEL, tally_dy=ny * SKIN2_VOXEL)
+
+
+def skin2_scene(p: np.ndarray, photons: int = 100_000, seed: int = 1) -> Scene:
+    """One parameter vector as a materialised voxel-grid Scene (for the oracle and for tests)."""
+    import ymc3d_amd as y
+
+    props, geo = skin2_tables(p)
+    rows = y.layered_voxelize(geo[0], SKIN2_SHAPE)
+    vals = np.where(rows == 0, 255, rows.astype(np.int32) - 1)
+    rec = np.stack([props[0][0, 1:], props[1][0, 1:], props[2][0, 1:], props[3][0, 1:]], axis=1)[None].astype(np.float64)
+    params = dict(skin2_params(photons), seed=seed)
+    return Scene("skin2_vessel_128x128x64", vals.astype(np.int32), (SKIN2_VOXEL,) * 3, rec, params)

@@ -300,6 +300,31 @@ void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Parametric layered geometry as a voxel grid: the same binary32 cylinder test as the device
+// (each product and sum a separate statement, so nothing is contracted).
+
+ymc_status layered_voxelize(const ymc_layered *g, int nx, int ny, int nz, uint8_t *rows) {
+  if (!g || !rows || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "layered_voxelize: bad arguments");
+  const float r2 = g->vessel_r * g->vessel_r;
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) {
+      int row = k < g->top_voxels ? g->row_top : g->row_bottom;
+      if (g->row_vessel >= 0) {
+        float ci = (float)i + 0.5f;
+        float ck = (float)k + 0.5f;
+        float fx = ci - g->vessel_x;
+        float fz = ck - g->vessel_z;
+        float ax = fx * fx;
+        float az = fz * fz;
+        float d2 = ax + az;
+        if (d2 <= r2) row = g->row_vessel;
+      }
+      for (int j = 0; j < ny; ++j) rows[((size_t)k * ny + j) * nx + i] = (uint8_t)row;
+    }
+  return YMC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
 // Binary formats, src/mc3d_io.cu.
 
 namespace {
@@ -460,5 +485,8 @@ ymc_status ymc_write_property(const char *path, int32_t dims, int32_t nx, int32_
   return ymc::write_property(path, dims, nx, ny, nz, dx, dy, dz, data);
 }
 void ymc_free(void *p) { std::free(p); }
+ymc_status ymc_layered_voxelize(const ymc_layered *g, int32_t nx, int32_t ny, int32_t nz, uint8_t *rows) {
+  return ymc::layered_voxelize(g, nx, ny, nz, rows);
+}
 
 }  // extern "C"

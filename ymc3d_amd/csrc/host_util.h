@@ -12,6 +12,7 @@ const char *last_error();
 constexpr int kSafeMax = 31;
 void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius);
 
+ymc_status layered_voxelize(const ymc_layered *g, int nx, int ny, int nz, uint8_t *rows);
 ymc_status generate_multipliers(uint32_t count, uint32_t *out);
 ymc_status read_multipliers_text(const char *path, uint32_t *out, uint32_t cap, uint32_t *count);
 ymc_status seed_streams(const uint32_t *table, uint32_t table_len, uint64_t seed, uint64_t n, uint64_t *x_out,

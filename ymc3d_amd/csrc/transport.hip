@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/ymc3d.h"
@@ -98,6 +99,8 @@ struct KArgs {
   const uint16_t *vox;
   int nx, ny, nz;
   float dx, dy, dz, Lx, Ly;
+  // parametric geometry per table (layered runs only)
+  const ymc_layered *layered;
   // optical properties
   const RowConst *rows;  // [num_tables * num_types]
   const float *w0;       // [num_tables] launch weight 1 - Rsp
@@ -165,13 +168,45 @@ __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int
   return a.vox[((size_t)kc * a.ny + jc) * a.nx + ic];
 }
 
+// Voxel word of a parametric geometry (ymc_layered): the row of voxel (i, j, k) of the grid the
+// parameters describe (same clamp as the texture), and a safe radius bounded from the distances
+// to the layer interface, the cylinder surface and the walls.  The cylinder test is evaluated
+// with every binary32 operation rounded, exactly as the host voxeliser does.
+__device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layered &g, int i, int j, int k) {
+  const int ic = min(max(i, 0), a.nx - 1);
+  const int jc = min(max(j, 0), a.ny - 1);
+  const int kc = min(max(k, 0), a.nz - 1);
+  const bool top = kc < g.top_voxels;
+  int row = top ? g.row_top : g.row_bottom;
+  int r = top ? min(kc, g.top_voxels - 1 - kc) : kc - g.top_voxels;
+  if (g.row_vessel >= 0) {
+    const float fx = __fsub_rn(__fadd_rn((float)ic, 0.5f), g.vessel_x);
+    const float fz = __fsub_rn(__fadd_rn((float)kc, 0.5f), g.vessel_z);
+    const float d2 = __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fz, fz));
+    if (d2 <= __fmul_rn(g.vessel_r, g.vessel_r)) row = g.row_vessel;
+    // voxels within Chebyshev distance r have centres within r*sqrt(2) in the xz plane
+    const float gap = fabsf(sqrtf(d2) - g.vessel_r) - 0.01f;
+    r = min(r, gap > 0.0f ? (int)(gap * 0.70710678f) : 0);
+  }
+  r = min(r, min(min(ic, a.nx - 1 - ic), min(jc, a.ny - 1 - jc)));
+  r = min(max(r, 0), kSafeMax);
+  return (uint32_t)row | ((uint32_t)r << 8);
+}
+
+template <bool LAYERED>
+__device__ __forceinline__ uint32_t geometry_word(const KArgs &a, const ymc_layered *g, int i, int j, int k) {
+  if (LAYERED) return layered_word(a, *g, i, j, k);
+  return voxel_word(a, i, j, k);
+}
+
 struct LaneAcc {
   uint32_t steps, fresnel, photons;
   uint64_t absorbed;
 };
 
-template <bool UNIFORM>
-__device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float w0) {
+template <bool UNIFORM, bool LAYERED>
+__device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float w0,
+                                              const ymc_layered *lg) {
   if (UNIFORM) {
     p.x = a.Lx * u32_co(mwc_next(rx, ra));  // src/mc3d_gpu.cu:88
     p.y = a.Ly * u32_co(mwc_next(rx, ra));  // :89
@@ -191,7 +226,7 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
   p.i = grid_coord(p.x, 0.0f, a.dx);
   p.j = grid_coord(p.y, 0.0f, a.dy);
   p.k = 0;                                         // grid_coord(0, +1) = 0
-  const uint32_t w = voxel_word(a, p.i, p.j, 0);   // uz > 0: never the air override
+  const uint32_t w = geometry_word<LAYERED>(a, lg, p.i, p.j, 0);  // uz > 0: never the air override
   p.t = (int)(w & 0xffu);
   p.nstep = 0;
   p.ai = p.i;
@@ -247,9 +282,10 @@ __device__ __forceinline__ bool fast_crossing(Packet &p, uint64_t &rx, uint32_t 
 enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
 
 // One step of a live packet.
-template <bool COUNTERS>
+template <bool COUNTERS, bool LAYERED>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
-                                            const RowConst *tab, unsigned long long *tot, LaneAcc &acc) {
+                                            const RowConst *tab, unsigned long long *tot, LaneAcc &acc,
+                                            const ymc_layered *lg) {
   const float4 c0 = tab[p.t].r0;
   const float4 c1 = tab[p.t].r1;
   const float ni = c0.w;
@@ -277,7 +313,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const int ni_ = p.i + (p.ux >= 0.0f ? (nx >= bx ? 1 : 0) : (nx <= bx ? -1 : 0));
   const int nj_ = p.j + (p.uy >= 0.0f ? (ny >= by ? 1 : 0) : (ny <= by ? -1 : 0));
   const int nk_ = p.k + (p.uz >= 0.0f ? (nz >= bz ? 1 : 0) : (nz <= bz ? -1 : 0));
-  const uint32_t nw = voxel_word(a, ni_, nj_, nk_);
+  const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
   const int next_t = air_override ? YMC_TISSUE_AIR : (int)(nw & 0xffu);
   const float nt = tab[next_t].r0.w;
@@ -472,27 +508,32 @@ __host__ __device__ inline size_t lds_tot_offset(int tables, int types) {
   size_t off = (size_t)tables * types * sizeof(RowConst) + (size_t)tables * sizeof(float);
   return (off + 7) & ~(size_t)7;
 }
-__host__ __device__ inline size_t lds_bytes(int tables, int types) {
-  return lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
+__host__ __device__ inline size_t lds_bytes(int tables, int types, bool layered) {
+  return layered ? 0 : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
 }
 
-template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD>
+template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD, bool LAYERED>
 __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  // LDS: optical-property rows, launch weights, then the workgroup's per-table totals, which
-  // are accumulated with LDS atomics and added to the device totals once, when the block exits.
+  // Voxel-grid runs (few tables, many packets each) keep the optical-property rows, launch
+  // weights and the workgroup's per-table totals in LDS; the totals are accumulated with LDS
+  // atomics and added to the device totals once, when the block exits.  Parametric-geometry runs
+  // (one table per sample, thousands of tables, few packets each) read rows, weights and
+  // geometries from global memory (cached) and add their totals there directly.
   RowConst *s_rows = reinterpret_cast<RowConst *>(lds_raw);
   float *s_w0 = reinterpret_cast<float *>(s_rows + a.num_tables * a.num_types);
   unsigned long long *s_tot = reinterpret_cast<unsigned long long *>(lds_raw + lds_tot_offset(a.num_tables, a.num_types));
-  for (int q = threadIdx.x; q < a.num_tables * kTotFields; q += blockDim.x) s_tot[q] = 0;
-  {
+  if (!LAYERED) {
+    for (int q = threadIdx.x; q < a.num_tables * kTotFields; q += blockDim.x) s_tot[q] = 0;
     const int nrow = a.num_tables * a.num_types;
     const float4 *src = reinterpret_cast<const float4 *>(a.rows);
     float4 *dst = reinterpret_cast<float4 *>(s_rows);
     for (int q = threadIdx.x; q < nrow * 3; q += blockDim.x) dst[q] = src[q];
     for (int q = threadIdx.x; q < a.num_tables; q += blockDim.x) s_w0[q] = a.w0[q];
+    __syncthreads();
   }
-  __syncthreads();
+  const RowConst *rows = LAYERED ? a.rows : s_rows;
+  unsigned long long *totals = LAYERED ? a.tot : s_tot;
 
   const uint32_t lane = __lane_id();
   Packet p;
@@ -517,14 +558,14 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       if (left == 0 && !drained) {
         // Chain finished: flush its counters, then take the next chain.  One atomic per wave.
         if (COUNTERS && acc.photons) {
-          unsigned long long *tt = s_tot + (size_t)table * kTotFields;
+          unsigned long long *tt = totals + (size_t)table * kTotFields;
           atomicAdd(&tt[kPhotons], (unsigned long long)acc.photons);
           atomicAdd(&tt[kSteps], (unsigned long long)acc.steps);
           atomicAdd(&tt[kFresnel], (unsigned long long)acc.fresnel);
           atomicAdd(&tt[kAw], (unsigned long long)acc.absorbed);
           acc = LaneAcc{0, 0, 0, 0};
         } else if (!COUNTERS && acc.photons) {
-          atomicAdd(&s_tot[(size_t)table * kTotFields + kPhotons], (unsigned long long)acc.photons);
+          atomicAdd(&totals[(size_t)table * kTotFields + kPhotons], (unsigned long long)acc.photons);
           acc.photons = 0;
         }
         const uint64_t want = __ballot(1);
@@ -547,14 +588,14 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
         }
       }
       if (left > 0) {
-        launch_packet<UNIFORM>(p, rx, ra, a, s_w0[table]);
+        launch_packet<UNIFORM, LAYERED>(p, rx, ra, a, LAYERED ? a.w0[table] : s_w0[table], a.layered + table);
         alive = true;
       }
     }
     if (__ballot(alive) == 0) break;
     // Plain crossings inside safe cubes, while enough lanes of the wave have one.
     if (a.fast_min <= 64) {
-      const float mu_t = alive ? s_rows[(size_t)table * a.num_types + p.t].r1.x : 0.0f;
+      const float mu_t = alive ? rows[(size_t)table * a.num_types + p.t].r1.x : 0.0f;
       for (int it = 0; it < a.fast_max; ++it) {
         bool f = false;
         if (alive) f = fast_crossing<COUNTERS>(p, rx, ra, a, mu_t, acc);
@@ -562,9 +603,9 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       }
     }
     if (alive) {
-      const RowConst *tab = s_rows + (size_t)table * a.num_types;
-      unsigned long long *tt = s_tot + (size_t)table * kTotFields;
-      const int res = step_packet<COUNTERS>(p, rx, ra, a, tab, tt, acc);
+      const RowConst *tab = rows + (size_t)table * a.num_types;
+      unsigned long long *tt = totals + (size_t)table * kTotFields;
+      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, a.layered + table);
       if (res != kLive) {
         tally_packet<RADIAL>(p, a, table, tt);
         if (RECORD) {
@@ -587,9 +628,10 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       }
     }
   }
-  // Final flush: lane counters into LDS, then the block's totals into the device totals.
+  // Final flush: lane counters into the totals, then (LDS totals) the block's totals into the
+  // device totals.
   if (acc.photons) {
-    unsigned long long *tt = s_tot + (size_t)table * kTotFields;
+    unsigned long long *tt = totals + (size_t)table * kTotFields;
     atomicAdd(&tt[kPhotons], (unsigned long long)acc.photons);
     if (COUNTERS) {
       atomicAdd(&tt[kSteps], (unsigned long long)acc.steps);
@@ -597,10 +639,12 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       atomicAdd(&tt[kAw], (unsigned long long)acc.absorbed);
     }
   }
-  __syncthreads();
-  for (int q = threadIdx.x; q < a.num_tables * kTotFields; q += blockDim.x) {
-    const unsigned long long v = s_tot[q];
-    if (v) atomicAdd(&a.tot[q], v);
+  if (!LAYERED) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < a.num_tables * kTotFields; q += blockDim.x) {
+      const unsigned long long v = s_tot[q];
+      if (v) atomicAdd(&a.tot[q], v);
+    }
   }
 }
 
@@ -667,6 +711,10 @@ struct ymc_engine {
   int nx = 0, ny = 0, nz = 0;
   float dx = 0, dy = 0, dz = 0;
   bool have_geom = false;
+  // parametric geometry (ymc_engine_set_layered) instead of a voxel grid
+  bool layered = false;
+  ymc::DevBuf layered_buf;
+  std::vector<ymc_layered> layered_host;
   // optprops
   ymc::DevBuf rows, w0;
   int num_types = 0, num_tables = 0;
@@ -710,31 +758,21 @@ static ymc_status check_engine(ymc_engine *e) {
   return YMC_OK;
 }
 
-template <bool U, bool C, bool R, bool REC>
+template <int KEY>
 static const void *kernel_ptr() {
-  return reinterpret_cast<const void *>(&transport_kernel<U, C, R, REC>);
+  return reinterpret_cast<const void *>(
+      &transport_kernel<(KEY & 16) != 0, (KEY & 8) != 0, (KEY & 4) != 0, (KEY & 2) != 0, (KEY & 1) != 0>);
 }
 
-static const void *select_kernel(bool uniform, bool counters, bool radial, bool record) {
-  int key = (uniform ? 8 : 0) | (counters ? 4 : 0) | (radial ? 2 : 0) | (record ? 1 : 0);
-  switch (key) {
-    case 0: return kernel_ptr<false, false, false, false>();
-    case 1: return kernel_ptr<false, false, false, true>();
-    case 2: return kernel_ptr<false, false, true, false>();
-    case 3: return kernel_ptr<false, false, true, true>();
-    case 4: return kernel_ptr<false, true, false, false>();
-    case 5: return kernel_ptr<false, true, false, true>();
-    case 6: return kernel_ptr<false, true, true, false>();
-    case 7: return kernel_ptr<false, true, true, true>();
-    case 8: return kernel_ptr<true, false, false, false>();
-    case 9: return kernel_ptr<true, false, false, true>();
-    case 10: return kernel_ptr<true, false, true, false>();
-    case 11: return kernel_ptr<true, false, true, true>();
-    case 12: return kernel_ptr<true, true, false, false>();
-    case 13: return kernel_ptr<true, true, false, true>();
-    case 14: return kernel_ptr<true, true, true, false>();
-    default: return kernel_ptr<true, true, true, true>();
-  }
+template <int... K>
+static const void *pick(int key, std::integer_sequence<int, K...>) {
+  static const void *const table[] = {kernel_ptr<K>()...};
+  return table[key];
+}
+
+static const void *select_kernel(bool uniform, bool counters, bool radial, bool record, bool layered) {
+  const int key = (uniform ? 16 : 0) | (counters ? 8 : 0) | (radial ? 4 : 0) | (record ? 2 : 0) | (layered ? 1 : 0);
+  return pick(key, std::make_integer_sequence<int, 32>{});
 }
 
 }  // namespace ymc
@@ -789,6 +827,33 @@ void ymc_engine_destroy(ymc_engine *e) {
   delete e;  // DevBuf destructors free device memory
 }
 
+ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t nz, float dx, float dy, float dz,
+                                  const ymc_layered *per_table, int32_t num_tables) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!per_table || num_tables < 1 || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0) || !(dy > 0) || !(dz > 0))
+    return fail(YMC_ERR_ARG, "ymc_engine_set_layered: invalid arguments");
+  for (int w = 0; w < num_tables; ++w) {
+    const ymc_layered &g = per_table[w];
+    if (g.top_voxels < 0 || g.row_top < 0 || g.row_top > 255 || g.row_bottom < 0 || g.row_bottom > 255 ||
+        g.row_vessel > 255 || !(g.vessel_r >= 0) || !std::isfinite(g.vessel_x) || !std::isfinite(g.vessel_z))
+      return fail(YMC_ERR_ARG, "layered geometry %d invalid", w);
+  }
+  if ((st = e->layered_buf.ensure(num_tables * sizeof(ymc_layered)))) return st;
+  HIP_TRY(hipMemcpy(e->layered_buf.p, per_table, num_tables * sizeof(ymc_layered), hipMemcpyHostToDevice));
+  e->layered_host.assign(per_table, per_table + num_tables);
+  e->nx = nx;
+  e->ny = ny;
+  e->nz = nz;
+  e->dx = dx;
+  e->dy = dy;
+  e->dz = dz;
+  e->have_geom = true;
+  e->layered = true;
+  e->prepared = false;
+  return YMC_OK;
+}
+
 ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *out) {
   if (!rows || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "ymc_safe_radius: bad arguments");
   safe_radius(rows, nx, ny, nz, out);
@@ -818,6 +883,7 @@ ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
   for (size_t v = 0; v < nv; ++v) mx = std::max(mx, g->tissue[v]);
   e->host_n.assign(1, (float)mx);
   e->have_geom = true;
+  e->layered = false;
   e->prepared = false;
   return YMC_OK;
 }
@@ -861,8 +927,18 @@ ymc_status ymc_engine_set_optprops(ymc_engine *e, const ymc_optprops *p) {
 static ymc_status prepare_common(ymc_engine *e, const ymc_params *prm) {
   if (!e->have_geom || !e->have_props) return fail(YMC_ERR_STATE, "set geometry and optical properties first");
   if (!prm) return fail(YMC_ERR_ARG, "null params");
-  if (!e->host_n.empty() && e->host_n[0] >= (float)e->num_types)
+  if (!e->layered && !e->host_n.empty() && e->host_n[0] >= (float)e->num_types)
     return fail(YMC_ERR_FORMAT, "geometry uses row %d but the tables have %d rows", (int)e->host_n[0], e->num_types);
+  if (e->layered) {
+    if ((int)e->layered_host.size() != e->num_tables)
+      return fail(YMC_ERR_ARG, "layered geometry has %zu tables, optical properties %d", e->layered_host.size(),
+                  e->num_tables);
+    for (const ymc_layered &g : e->layered_host) {
+      if (g.row_top < 0 || g.row_top >= e->num_types || g.row_bottom < 0 || g.row_bottom >= e->num_types ||
+          g.row_vessel >= e->num_types)
+        return fail(YMC_ERR_FORMAT, "layered geometry names a row outside the %d-row tables", e->num_types);
+    }
+  }
   e->params = *prm;
   e->tnx = prm->tally_nx > 0 ? prm->tally_nx : e->nx;
   e->tny = prm->tally_ny > 0 ? prm->tally_ny : e->ny;
@@ -959,6 +1035,7 @@ ymc_status ymc_engine_run(ymc_engine *e) {
 
   KArgs a{};
   a.vox = e->vox.as<uint16_t>();
+  a.layered = e->layered_buf.as<ymc_layered>();
   a.nx = e->nx;
   a.ny = e->ny;
   a.nz = e->nz;
@@ -1011,8 +1088,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   const bool uniform = prm.illumination == YMC_ILLUM_UNIFORM;
   const bool counters = (prm.flags & YMC_FLAG_COUNTERS) != 0;
   const bool record = (prm.flags & YMC_FLAG_RECORD) != 0;
-  const void *kern = select_kernel(uniform, counters, e->nr > 0, record);
-  const size_t lds = lds_bytes(e->num_tables, e->num_types);
+  const void *kern = select_kernel(uniform, counters, e->nr > 0, record, e->layered);
+  const size_t lds = lds_bytes(e->num_tables, e->num_types, e->layered);
   if (lds > 64 * 1024) return fail(YMC_ERR_ARG, "optical-property tables need %zu B of LDS (max 65536)", lds);
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds));

@@ -206,6 +206,14 @@ class Engine:
         check(self._lib.ymc_engine_set_geometry(self._h, C.byref(g)), "ymc_engine_set_geometry")
         self._shape = (nx, ny, nz, float(dx), float(dy), float(dz))
 
+    def set_layered(self, per_table: np.ndarray, shape: tuple[int, int, int], voxel: tuple[float, float, float]) -> None:
+        """Parametric geometry per table (structured LAYERED_DTYPE array); shape is (nx, ny, nz)."""
+        g = np.ascontiguousarray(per_table, dtype=_abi.LAYERED_DTYPE)
+        nx, ny, nz = shape
+        check(self._lib.ymc_engine_set_layered(self._h, nx, ny, nz, *voxel, g.ctypes.data_as(C.c_void_p), g.size),
+              "ymc_engine_set_layered")
+        self._shape = (nx, ny, nz, *map(float, voxel))
+
     def set_optprops(self, mua, mus, g, n) -> None:
         """Arrays [types] or [tables, types]; row 0 is the ambient medium."""
         arrs = [_as_tables(v, k) for v, k in ((mua, "mua"), (mus, "mus"), (g, "g"), (n, "n"))]
@@ -309,6 +317,16 @@ class Engine:
         check(self._lib.ymc_engine_rng_fill(self._h, ptr(xx, C.c_uint64), ptr(aa, C.c_uint32), xx.size, draws, kind,
                                             ptr(out, C.c_float), ptr(xf, C.c_uint64)), "ymc_engine_rng_fill")
         return out, xf
+
+
+def layered_voxelize(geom, shape: tuple[int, int, int]) -> np.ndarray:
+    """Rows [nz, ny, nx] of the voxel grid one parametric geometry describes."""
+    g = np.ascontiguousarray(np.atleast_1d(geom)[:1], dtype=_abi.LAYERED_DTYPE)
+    nx, ny, nz = shape
+    rows = np.zeros((nz, ny, nx), dtype=np.uint8)
+    check(load_library().ymc_layered_voxelize(g.ctypes.data_as(C.c_void_p), nx, ny, nz, ptr(rows, C.c_uint8)),
+          "ymc_layered_voxelize")
+    return rows
 
 
 def run_3dmc_gpu(tissue_rows: np.ndarray, dx: float, dy: float, dz: float, mua, mus, g, n, num_photons: int,

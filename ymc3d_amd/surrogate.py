@@ -1,0 +1,139 @@
+"""Reflectance surrogate: an ensemble of MLPs 12 -> 256 -> 256 -> 256 -> 33 (GELU) mapping the
+twelve skin parameters of configs.sample_skin2_params to log10 of 32 radial reflectance bins and of
+the total diffuse reflectance.  Training and inference run on the device (libymc3d.so, BF16
+matrix-core kernels with FP32 accumulation, Adam on FP32 master weights).  The spread of the
+ensemble members is the error estimate; queries above threshold, or outside the training ranges,
+fall back to Monte Carlo.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import configs
+from ._abi import ptr
+from .engine import check, load_library
+
+WIDTHS = (12, 256, 256, 256, 33)
+LOG_FLOOR = 1e-7  # reflectance per packet below this is treated as this when taking log10
+
+
+class MLP:
+    """One network on the device (ymc_mlp)."""
+
+    def __init__(self, widths=WIDTHS, seed: int = 0, device: int = 0):
+        self._lib = load_library()
+        self.widths = tuple(int(w) for w in widths)
+        w = np.array(self.widths, np.int32)
+        self._h = C.c_void_p()
+        check(self._lib.ymc_mlp_create(device, ptr(w, C.c_int32), w.size, seed, C.byref(self._h)), "ymc_mlp_create")
+        self.n_params = int(self._lib.ymc_mlp_num_params(self._h))
+
+    def close(self):
+        if self._h:
+            self._lib.ymc_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        check(self._lib.ymc_mlp_get_params(self._h, ptr(p, C.c_float), p.size), "ymc_mlp_get_params")
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        check(self._lib.ymc_mlp_set_params(self._h, ptr(p, C.c_float), p.size), "ymc_mlp_set_params")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((x.shape[0], self.widths[-1]), np.float32)
+        check(self._lib.ymc_mlp_forward(self._h, ptr(x, C.c_float), x.shape[0], ptr(y, C.c_float)), "ymc_mlp_forward")
+        return y
+
+    def gradients(self, x: np.ndarray, y: np.ndarray) -> tuple[float, np.ndarray]:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        check(self._lib.ymc_mlp_gradients(self._h, ptr(x, C.c_float), ptr(y, C.c_float), x.shape[0],
+                                          ptr(g, C.c_float), C.byref(loss)), "ymc_mlp_gradients")
+        return loss.value, g
+
+    def fit(self, x: np.ndarray, y: np.ndarray, epochs: int, batch: int = 512, lr: float = 2e-3,
+            lr_final: float = 2e-5, seed: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        check(self._lib.ymc_mlp_set_data(self._h, ptr(x, C.c_float), ptr(y, C.c_float), x.shape[0]),
+              "ymc_mlp_set_data")
+        losses = np.zeros(epochs, np.float32)
+        check(self._lib.ymc_mlp_train(self._h, epochs, batch, lr, lr_final, seed, ptr(losses, C.c_float)),
+              "ymc_mlp_train")
+        return losses
+
+
+def encode_params(P: np.ndarray) -> np.ndarray:
+    """Parameters to network inputs in [-1, 1] over the training ranges (log scale where sampled so)."""
+    P = np.atleast_2d(np.asarray(P, np.float64))
+    lo, hi = configs.PARAM_LO, configs.PARAM_HI
+    v = np.where(configs.PARAM_LOG, np.log(np.maximum(P, 1e-30)), P)
+    a = np.where(configs.PARAM_LOG, np.log(lo), lo)
+    b = np.where(configs.PARAM_LOG, np.log(hi), hi)
+    return (2.0 * (v - a) / (b - a) - 1.0).astype(np.float32)
+
+
+def in_range(P: np.ndarray) -> np.ndarray:
+    P = np.atleast_2d(P)
+    return np.all((P >= configs.PARAM_LO * (1 - 1e-6)) & (P <= configs.PARAM_HI * (1 + 1e-6)), axis=1)
+
+
+def encode_targets(radial: np.ndarray, total: np.ndarray) -> np.ndarray:
+    """log10 of the 32 radial bins and the total reflectance (per packet)."""
+    return np.log10(np.maximum(np.concatenate([radial, total[:, None]], axis=1), LOG_FLOOR)).astype(np.float32)
+
+
+@dataclass
+class Ensemble:
+    members: list = field(default_factory=list)
+    y_mean: np.ndarray | None = None
+    y_std: np.ndarray | None = None
+
+    @classmethod
+    def train(cls, P: np.ndarray, targets: np.ndarray, members: int = 4, epochs: int = 60, batch: int = 512,
+              lr: float = 2e-3, seed: int = 0, verbose: bool = False):
+        x = encode_params(P)
+        ens = cls()
+        ens.y_mean = targets.mean(0)
+        ens.y_std = targets.std(0) + 1e-6
+        yz = ((targets - ens.y_mean) / ens.y_std).astype(np.float32)
+        for m in range(members):
+            net = MLP(seed=seed * 1000 + m)
+            # each member sees its own bootstrap resample of the data
+            idx = np.random.default_rng(seed * 1000 + m).integers(0, len(x), len(x))
+            losses = net.fit(x[idx], yz[idx], epochs, batch, lr, lr * 0.01, seed * 1000 + m)
+            if verbose:
+                print(f"member {m}: loss {losses[0]:.4f} -> {losses[-1]:.5f}", flush=True)
+            ens.members.append(net)
+        return ens
+
+    def predict(self, P: np.ndarray, chunk: int = 1 << 18) -> tuple[np.ndarray, np.ndarray]:
+        """Mean prediction (log10 targets) and the member standard deviation (log10 units)."""
+        P = np.atleast_2d(P)
+        mean = np.empty((P.shape[0], WIDTHS[-1]), np.float32)
+        spread = np.empty_like(mean)
+        for r0 in range(0, P.shape[0], chunk):
+            x = encode_params(P[r0:r0 + chunk])
+            outs = np.stack([m.forward(x) * self.y_std + self.y_mean for m in self.members])
+            mean[r0:r0 + chunk] = outs.mean(0)
+            spread[r0:r0 + chunk] = outs.std(0)
+        return mean, spread
+
+    def close(self):
+        for m in self.members:
+            m.close()
