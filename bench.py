@@ -29,6 +29,12 @@ FLOPS_PER_STEP = 108
 FLOPS_PER_FRESNEL = 31
 PEAK_FP32_TFLOPS = 157.3
 
+# HBM traffic per launch of the timed kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this
+# script's default workload, scaled by the same counters read on a 1 GiB device copy (x2.000 for
+# FETCH_SIZE, x1.000 for WRITE_SIZE); profiles/transport_skin_vessels_256x256x128.md.  The launch
+# is deterministic (same chains and streams every step), so the figure holds for every default run.
+MEASURED_TRAFFIC = {("skin_vessels", 100_000_000, 458752): 1.3968e12}
+
 WORKLOADS = {
     "skin_vessels": ("skin_vessels", 100_000_000),
     "skin_slab_sphere": ("skin_slab_sphere", 1_000_000),
@@ -47,7 +53,12 @@ def main() -> int:
     ap.add_argument("--chain-lanes", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="instead of the workload, time a plain device-to-device copy of this many bytes (the "
+                         "reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        return copy_reference(args.copy_bytes, max(args.steps, 1))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the photon budget shards across GPUs with one integer all-reduce of "
@@ -138,7 +149,7 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.workload, n_photons, eng.pool_lanes())),
         },
         "cpu_baseline": None,
     }
@@ -147,6 +158,24 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, tab, params, n_photons, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def copy_reference(nbytes: int, reps: int) -> int:
+    """Device-to-device copy of nbytes (one read + one write of nbytes per launch), reps launches."""
+    import torch
+
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes": nbytes, "launches": reps + 2, "ms_per_copy": dt * 1e3,
+                      "GBps_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
     return 0
 
 
