@@ -132,3 +132,18 @@ def test_kill_rules_are_rare(multipliers):
     # which would kill every packet whose first scatter turns it back across the beam's face
     r2 = O.run_scene(s, multipliers, kill_zero_step=2)
     assert r2.totals[0]["killed"] > 100 * max(1, int(t["killed"]))
+
+
+def test_roulette_is_unbiased(multipliers):
+    """Russian roulette (src/mc3d_gpu.cu:275-277) must not change the expected weight: a packet
+    below 1e-4 survives with p = 0.1 at ten times its weight, so each event changes the tallied
+    weight by +9W or -W (W < 1e-4, mean 0, standard deviation <= 3e-4).  The net roulette weight
+    of 50 000 packets (about 1.1 events each in this absorbing slab) stays within 5 of those
+    standard deviations of zero; a survivor left at W, or a survival chance off by 2x, would shift
+    it by more than 2."""
+    s = configs.skin_slab_sphere(photons=50_000)
+    r = O.run_scene(s, multipliers)
+    t = r.totals[0]
+    net = int(t["roulette_w"]) / FX
+    assert abs(net) < 5 * 3e-4 * np.sqrt(2.0 * t["photons"]), net
+    assert t["A_w"] > 0

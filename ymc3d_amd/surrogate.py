@@ -137,3 +137,36 @@ class Ensemble:
     def close(self):
         for m in self.members:
             m.close()
+
+
+def serve(ens: Ensemble, P: np.ndarray, engine, threshold: float, fallback_photons: int, audit_frac: float = 0.0,
+          seed: int = 0, **mc_kw) -> dict:
+    """Answers parameter rows P with the surrogate, falling back to Monte Carlo.
+
+    A row is re-run by batched Monte Carlo (`fallback_photons` packets, surrogate_data.run_mc) when it
+    lies outside the training ranges or when the largest ensemble standard deviation over its 33
+    outputs exceeds `threshold` (log10 units).  A seeded `audit_frac` share of the other rows is
+    re-run too, to measure the served error; those keep the surrogate answer.  Returns `pred`
+    (log10 targets, Monte Carlo values in flagged rows), `flagged`, `audit`, `rows` (indices
+    re-run), `mc` (the run_mc result for `rows`) and `surrogate_s` (inference seconds)."""
+    import time
+
+    from . import surrogate_data
+
+    P = np.atleast_2d(P)
+    t0 = time.perf_counter()
+    mean, spread = ens.predict(P)
+    t_sur = time.perf_counter() - t0
+    flagged = (~in_range(P)) | (spread.max(1) > threshold)
+    audit = np.zeros(len(P), bool)
+    if audit_frac > 0:
+        audit = (~flagged) & (np.random.default_rng(seed).uniform(size=len(P)) < audit_frac)
+    rows = np.flatnonzero(flagged | audit)
+    mc = None
+    pred = mean.copy()
+    if rows.size:
+        mc = surrogate_data.run_mc(engine, P[rows], fallback_photons, seed=seed + 1, **mc_kw)
+        fb = flagged[rows]
+        pred[rows[fb]] = encode_targets(mc["radial"][fb], mc["total"][fb])
+    return {"pred": pred, "mean": mean, "spread": spread, "flagged": flagged, "audit": audit, "rows": rows,
+            "mc": mc, "surrogate_s": t_sur}

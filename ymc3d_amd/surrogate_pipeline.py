@@ -45,7 +45,7 @@ def main(argv=None) -> int:
     ap.add_argument("--test", type=int, default=1024)
     ap.add_argument("--test-photons", type=int, default=1_000_000)
     ap.add_argument("--cube", type=int, nargs=2, default=(256, 256))
-    ap.add_argument("--fallback-photons", type=int, default=100_000)
+    ap.add_argument("--fallback-photons", type=int, default=1_000_000)
     ap.add_argument("--spread-quantile", type=float, default=0.97)
     ap.add_argument("--out", default="surrogate_report.json")
     ap.add_argument("--seed", type=int, default=0)
@@ -96,18 +96,10 @@ def main(argv=None) -> int:
     # 4. serving a hyperspectral cube with Monte Carlo fallback
     H, W = a.cube
     Q, outl = D.hyperspectral_cube(H, W, 64, 0.05, seed=a.seed + 200)
-    t0 = time.perf_counter()
-    qmean, qspread = ens.predict(Q)
-    t_sur = time.perf_counter() - t0
+    sv = S.serve(ens, Q, eng, threshold, a.fallback_photons, audit_frac=0.01, seed=a.seed + 300,
+                 tables_per_launch=1024, progress_every=0.0, label="fallback-mc")
+    qmean, flag, audit, todo, mcq, t_sur = sv["mean"], sv["flagged"], sv["audit"], sv["rows"], sv["mc"], sv["surrogate_s"]
     inr = S.in_range(Q)
-    flag = (~inr) | (qspread.max(1) > threshold)
-    rng = np.random.default_rng(a.seed + 300)
-    audit = (~flag) & (rng.uniform(size=len(Q)) < 0.01)
-    todo = np.flatnonzero(flag | audit)
-    mcq = D.run_mc(eng, Q[todo], a.fallback_photons, seed=a.seed + 400, label="fallback-mc",
-                   tables_per_launch=1024, progress_every=0.0)
-    fb = np.zeros(len(Q), bool)
-    fb[todo] = True
     aud_idx = np.flatnonzero(audit[todo])
     aerr, aref = rel_err(qmean[todo][aud_idx], mcq["radial"][aud_idx], mcq["total"][aud_idx])
     # the audit Monte Carlo has noise of its own: its relative SE, and the surrogate error left after
