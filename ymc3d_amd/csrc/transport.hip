@@ -45,7 +45,11 @@ namespace ymc {
 
 // MWC step (gpumcml_rng/gpumcml_rng.cu:40) as one v_mad_u64_u32.
 __device__ __forceinline__ uint32_t mwc_next(uint64_t &x, uint32_t a) {
-  x = (uint64_t)(uint32_t)x * (uint64_t)a + (x >> 32);
+  // Written as the instruction itself: in some inlined contexts the compiler otherwise loses
+  // track of the zero upper half of the multiplier and issues a second 64-bit multiply-add.
+  uint64_t r;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"((uint32_t)x), "v"(a), "v"(x >> 32) : "vcc");
+  x = r;
   return (uint32_t)x;
 }
 
@@ -60,6 +64,8 @@ __device__ __forceinline__ float u32_co(uint32_t u) {
 __device__ __forceinline__ float u32_oc(uint32_t u) { return 1.0f - u32_co(u); }
 
 __device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+// -ln(v) for v in [2^-32, 1]: v_log_f32 (log2, no denormal range in this domain) times ln 2.
+__device__ __forceinline__ float neg_ln(float v) { return __builtin_amdgcn_logf(v) * -0.693147180559945309f; }
 
 // Fixed-point weight, scale 2^32, round to nearest even (0 <= w < 2^32 after scaling).
 __device__ __forceinline__ uint64_t fx(float w) {
@@ -151,8 +157,28 @@ struct Packet {
   float iux, iuy, iuz;
   int i, j, k, t;
   uint32_t nstep;  // steps taken by this packet
-  int ai, aj, ak, ar;  // anchor voxel and its safe radius (ar < 0: no safe cube)
+  // Safe cube of the anchor voxel (ai, aj, ak) with radius r, stored as oi = r - ai (likewise oj,
+  // ok) and r2 = 2r so that voxel (i, j, k) lies in the cube iff (unsigned)(i + oi) <= r2 on all
+  // three axes.  No cube: offsets of -2^30 and r2 = 0, which no index passes.
+  int oi, oj, ok;
+  uint32_t r2;
 };
+
+constexpr int kNoCube = -(1 << 30);
+
+__device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, int r) {
+  p.oi = r - i;
+  p.oj = r - j;
+  p.ok = r - k;
+  p.r2 = (uint32_t)(2 * r);
+}
+__device__ __forceinline__ void clear_anchor(Packet &p) {
+  p.oi = p.oj = p.ok = kNoCube;
+  p.r2 = 0;
+}
+__device__ __forceinline__ bool in_cube(const Packet &p, int i, int j, int k) {
+  return ((uint32_t)(i + p.oi) <= p.r2) & ((uint32_t)(j + p.oj) <= p.r2) & ((uint32_t)(k + p.ok) <= p.r2);
+}
 
 // Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
 // launch boundary (src/mc3d_gpu.cu:499-504), so its end-of-launch zero-step check (:343) sees a
@@ -161,11 +187,19 @@ constexpr uint32_t kLaunchSteps = 45;
 // Step cap per packet (a packet that reaches it is killed; see the kill rule in step_packet).
 constexpr uint32_t kMaxSteps = 1u << 24;
 
+// Grids are limited to 2^31 voxels at upload, so the clamped index fits 32-bit arithmetic.
 __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int k) {
-  int ic = min(max(i, 0), a.nx - 1);
-  int jc = min(max(j, 0), a.ny - 1);
-  int kc = min(max(k, 0), a.nz - 1);
-  return a.vox[((size_t)kc * a.ny + jc) * a.nx + ic];
+  const uint32_t ic = (uint32_t)min(max(i, 0), a.nx - 1);
+  const uint32_t jc = (uint32_t)min(max(j, 0), a.ny - 1);
+  const uint32_t kc = (uint32_t)min(max(k, 0), a.nz - 1);
+  return a.vox[(kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic];
+}
+
+// Index after a move: an axis steps when the new coordinate reached the face ahead (the face at
+// b, in the direction of travel).
+__device__ __forceinline__ int next_index(int i, float u, float n, float b) {
+  const bool up = u >= 0.0f;
+  return i + (int)(up & (n >= b)) - (int)(!up & (n <= b));
 }
 
 // Voxel word of a parametric geometry (ymc_layered): the row of voxel (i, j, k) of the grid the
@@ -222,17 +256,14 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
   p.iuy = INFINITY;
   p.iuz = 1.0f;
   p.W = w0;                                        // :101
-  p.s = -__logf(u32_oc(mwc_next(rx, ra)));         // :102
+  p.s = neg_ln(u32_oc(mwc_next(rx, ra)));         // :102
   p.i = grid_coord(p.x, 0.0f, a.dx);
   p.j = grid_coord(p.y, 0.0f, a.dy);
   p.k = 0;                                         // grid_coord(0, +1) = 0
   const uint32_t w = geometry_word<LAYERED>(a, lg, p.i, p.j, 0);  // uz > 0: never the air override
   p.t = (int)(w & 0xffu);
   p.nstep = 0;
-  p.ai = p.i;
-  p.aj = p.j;
-  p.ak = 0;
-  p.ar = (int)(w >> 8);
+  set_anchor(p, p.i, p.j, 0, (int)(w >> 8));
 }
 
 // A plain crossing of a voxel face inside the anchor's safe cube: hit, positive step length,
@@ -254,12 +285,12 @@ __device__ __forceinline__ bool fast_crossing(Packet &p, uint64_t &rx, uint32_t 
   const float nx = (db == tx) ? bx : fmaf(p.ux, db, p.x);
   const float ny = (db == ty) ? by : fmaf(p.uy, db, p.y);
   const float nz = (db == tz) ? bz : fmaf(p.uz, db, p.z);
-  const int ni = p.i + (p.ux >= 0.0f ? (nx >= bx ? 1 : 0) : (nx <= bx ? -1 : 0));
-  const int nj = p.j + (p.uy >= 0.0f ? (ny >= by ? 1 : 0) : (ny <= by ? -1 : 0));
-  const int nk = p.k + (p.uz >= 0.0f ? (nz >= bz ? 1 : 0) : (nz <= bz ? -1 : 0));
-  const bool ok = dbm <= p.s && db > 0.0f && mu_t > 0.0f && abs(ni - p.ai) <= p.ar && abs(nj - p.aj) <= p.ar &&
-                  abs(nk - p.ak) <= p.ar && nx > 0.0f && nx < a.Lx && ny > 0.0f && ny < a.Ly &&
-                  p.nstep + 1 < kMaxSteps;
+  const int ni = next_index(p.i, p.ux, nx, bx);
+  const int nj = next_index(p.j, p.uy, ny, by);
+  const int nk = next_index(p.k, p.uz, nz, bz);
+  // one predicate without short-circuit branches
+  const bool ok = (dbm <= p.s) & (db > 0.0f) & (mu_t > 0.0f) & in_cube(p, ni, nj, nk) & (nx > 0.0f) &
+                  (nx < a.Lx) & (ny > 0.0f) & (ny < a.Ly) & (p.nstep + 1 < kMaxSteps);
   if (ok) {
     p.x = nx;
     p.y = ny;
@@ -310,10 +341,13 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const float s_left = p.s - ds;                    // :207
 
   // Next voxel (:209-213): an axis steps when the new coordinate reached the face ahead.
-  const int ni_ = p.i + (p.ux >= 0.0f ? (nx >= bx ? 1 : 0) : (nx <= bx ? -1 : 0));
-  const int nj_ = p.j + (p.uy >= 0.0f ? (ny >= by ? 1 : 0) : (ny <= by ? -1 : 0));
-  const int nk_ = p.k + (p.uz >= 0.0f ? (nz >= bz ? 1 : 0) : (nz <= bz ? -1 : 0));
-  const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
+  const int ni_ = next_index(p.i, p.ux, nx, bx);
+  const int nj_ = next_index(p.j, p.uy, ny, by);
+  const int nk_ = next_index(p.k, p.uz, nz, bz);
+  // Inside the anchor's safe cube the voxel's row is the current row: no load.
+  const bool cube = false;
+  uint32_t nw = (uint32_t)p.t;
+  if (!cube) nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
   const int next_t = air_override ? YMC_TISSUE_AIR : (int)(nw & 0xffu);
   const float nt = tab[next_t].r0.w;
@@ -327,9 +361,9 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     float cos_ai = xi ? p.ux : (yj ? p.uy : p.uz);
     cos_ai = fminf(fabsf(cos_ai), 1.0f);
     const float ni_nt = ni * rcp(nt);
-    const float sin_ai = sqrtf((1.0f - cos_ai) * (1.0f + cos_ai));  // sin(acos(c)), well conditioned
+    const float sin_ai = __builtin_amdgcn_sqrtf((1.0f - cos_ai) * (1.0f + cos_ai));  // sin(acos(c)), well conditioned
     const float sin_at = ni_nt * sin_ai;
-    const float cos_at = sqrtf(1.0f - sin_at * sin_at);
+    const float cos_at = __builtin_amdgcn_sqrtf(1.0f - sin_at * sin_at);
     const float R1 = (sin_ai * cos_at - cos_ai * sin_at) * rcp(sin_ai * cos_at + cos_ai * sin_at);
     const float r2d = a.fresnel_exact ? (cos_ai * cos_at + sin_ai * sin_at) : (cos_ai * cos_at + sin_at * sin_at);
     const float R2 = (cos_ai * cos_at - sin_ai * sin_at) * rcp(r2d);
@@ -371,7 +405,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
       const float xi = u32_oc(u_cos);
       float ct = c2.x * rcp(fmaf(c2.z, xi, c2.y));
       ct = c1.w * (c2.w - ct * ct);
-      const float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+      const float st = __builtin_amdgcn_sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
       const float rev = u32_co(u_phi);  // phi / (2 pi)
       const float sp = __builtin_amdgcn_sinf(rev);
       const float cp = __builtin_amdgcn_cosf(rev);
@@ -388,7 +422,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
         nuy = st * sp;
         nuz = copysignf(ct, ct * uz);
       }
-      s_new = (W > 0.0f) ? -__logf(u32_oc(u_len)) : 0.0f;  // :319-321
+      s_new = (W > 0.0f) ? neg_ln(u32_oc(u_len)) : 0.0f;  // :319-321
     }
   }
 
@@ -434,11 +468,11 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.j = nj_;
     p.k = nk_;
     p.t = next_t;
-    // the committed voxel is the one just loaded: it becomes the anchor
-    p.ai = ni_;
-    p.aj = nj_;
-    p.ak = nk_;
-    p.ar = air_override ? -1 : (int)(nw >> 8);
+    // a voxel loaded outside the old cube becomes the anchor
+    if (air_override)
+      clear_anchor(p);
+    else if (!cube)
+      set_anchor(p, ni_, nj_, nk_, (int)(nw >> 8));
   }
   if (COUNTERS) acc.steps++;
 
@@ -448,11 +482,11 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   // voxel its indices name - an interaction rounded onto a mirror wall and the index commit put
   // the packet outside the grid, after which the wall test bounces it forever) and a step cap.
   p.nstep++;
-  const bool launch_end = (p.nstep % kLaunchSteps) == 0;
-  const bool zero_check = a.kill0 == 2 || (a.kill0 == 1 && launch_end);
+  bool zero_kill = false;
+  if (db == 0.0f)  // rare: the launch-boundary test runs only for lanes that need it
+    zero_kill = a.kill0 == 2 || (a.kill0 == 1 && (p.nstep % kLaunchSteps) == 0);
   const bool bad = !isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z) || !isfinite(p.s) || !isfinite(p.ux) ||
-                   !isfinite(p.uy) || !isfinite(p.uz) || (zero_check && db == 0.0f) || db < 0.0f ||
-                   p.nstep >= kMaxSteps;
+                   !isfinite(p.uy) || !isfinite(p.uz) || zero_kill || db < 0.0f || p.nstep >= kMaxSteps;
   if (bad) {
     if (COUNTERS) {
       atomicAdd(&tot[kKilled], 1ull);
@@ -831,7 +865,8 @@ ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t
                                   const ymc_layered *per_table, int32_t num_tables) {
   ymc_status st = check_engine(e);
   if (st) return st;
-  if (!per_table || num_tables < 1 || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0) || !(dy > 0) || !(dz > 0))
+  if (!per_table || num_tables < 1 || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0) || !(dy > 0) || !(dz > 0) ||
+      (size_t)nx * ny * nz >= (1ull << 31))
     return fail(YMC_ERR_ARG, "ymc_engine_set_layered: invalid arguments");
   for (int w = 0; w < num_tables; ++w) {
     const ymc_layered &g = per_table[w];
@@ -866,6 +901,7 @@ ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
   if (!g || !g->tissue || g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || !(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0))
     return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: invalid geometry");
   size_t nv = (size_t)g->nx * g->ny * g->nz;
+  if (nv >= (1ull << 31)) return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: more than 2^31 voxels");
   std::vector<uint8_t> radius(nv);
   safe_radius(g->tissue, g->nx, g->ny, g->nz, radius.data());
   std::vector<uint16_t> words(nv);
