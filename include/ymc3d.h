@@ -59,6 +59,9 @@ enum { YMC_FRESNEL_YMC3D = 0, YMC_FRESNEL_EXACT = 1 };
 /* Run flags. */
 #define YMC_FLAG_COUNTERS 0x1u /* collect steps / Fresnel / absorption / roulette counters */
 #define YMC_FLAG_RECORD 0x2u   /* write one ymc_photon_record per packet (test hook) */
+/* Tally the absorbed weight per voxel: the deposit the reference has disabled
+ * (src/mc3d_gpu.cu:265-272), written by save_abs_map (src/mc3d_io.cu:229-255). */
+#define YMC_FLAG_ABSORPTION 0x4u
 
 /* Voxel geometry.  Mirrors geometry_t (src/mc3d_types.h:38-43) after geometry_read_from_file
  * (src/mc3d_io.cu:107-169) has remapped the file values: file value v in 0..254 becomes row
@@ -242,6 +245,11 @@ ymc_status ymc_engine_prepare_chains(ymc_engine *e, const ymc_params *params,
 ymc_status ymc_engine_run(ymc_engine *e);
 /* Copies the device tallies into `out`. */
 ymc_status ymc_engine_read_tally(ymc_engine *e, ymc_tally *out);
+/* Copies the absorption grid of the last run made with YMC_FLAG_ABSORPTION: count =
+ * num_tables * nz * ny * nx entries [table][k][j][i] of fixed-point absorbed weight (raw / 2^32),
+ * deposited at the voxel an absorbing step started in, clamped to the grid like the tissue
+ * lookup.  Its sum equals the totals' A_w. */
+ymc_status ymc_engine_read_absorption(ymc_engine *e, uint64_t *out, uint64_t count);
 /* Copies per-packet records of the last run made with YMC_FLAG_RECORD. */
 ymc_status ymc_engine_read_records(ymc_engine *e, ymc_photon_record *out, uint64_t cap);
 /* Milliseconds of the last ymc_engine_run transport kernel, from HIP events on its stream. */

@@ -38,6 +38,9 @@ _SIGS = {
     "ymc_oracle_build_chains": (C.c_uint64, [P(C.c_uint64), C.c_int32, C.c_int32, C.c_void_p, C.c_uint64]),
     "ymc_oracle_run": (C.c_int32, [P(_abi.Geometry), P(_abi.OptProps), P(_abi.Params), C.c_void_p, P(C.c_uint64),
                                    P(C.c_uint32), C.c_uint64, P(_abi.Tally), C.c_void_p, C.c_uint64, C.c_int32]),
+    "ymc_oracle_run_abs": (C.c_int32, [P(_abi.Geometry), P(_abi.OptProps), P(_abi.Params), C.c_void_p,
+                                       P(C.c_uint64), P(C.c_uint32), C.c_uint64, P(_abi.Tally), C.c_void_p,
+                                       C.c_uint64, C.c_int32, C.c_void_p]),
     "ymc_oracle_max_threads": (C.c_int32, []),
 }
 
@@ -90,8 +93,10 @@ class OracleResult:
     pass
 
 
-def run(scene_rows, voxel, optprops, chains, x, a, records: int = 0, threads: int = 0, **params):
-    """Runs the oracle over explicit chains.  Returns an engine-shaped TallyResult (+ records)."""
+def run(scene_rows, voxel, optprops, chains, x, a, records: int = 0, threads: int = 0, absorption: bool = False,
+        **params):
+    """Runs the oracle over explicit chains.  Returns an engine-shaped TallyResult (+ records, and with
+    absorption=True the fixed-point absorption grid [tables, nz, ny, nx] as `.absorption`)."""
     from ymc3d_amd.engine import TallyResult
 
     rows = np.ascontiguousarray(scene_rows, dtype=np.uint8)
@@ -125,13 +130,16 @@ def run(scene_rows, voxel, optprops, chains, x, a, records: int = 0, threads: in
     xx = np.ascontiguousarray(x, dtype=np.uint64)
     aa = np.ascontiguousarray(a, dtype=np.uint32)
     rec = np.zeros(records, dtype=_abi.RECORD_DTYPE)
-    st = lib().ymc_oracle_run(C.byref(geo), C.byref(op), C.byref(prm), ch.ctypes.data_as(C.c_void_p),
-                              ptr(xx, C.c_uint64), ptr(aa, C.c_uint32), ch.size, C.byref(t),
-                              rec.ctypes.data_as(C.c_void_p) if records else None, records, threads)
+    grid = np.zeros((L, nz, ny, nx), dtype=np.uint64) if absorption else None
+    st = lib().ymc_oracle_run_abs(C.byref(geo), C.byref(op), C.byref(prm), ch.ctypes.data_as(C.c_void_p),
+                                  ptr(xx, C.c_uint64), ptr(aa, C.c_uint32), ch.size, C.byref(t),
+                                  rec.ctypes.data_as(C.c_void_p) if records else None, records, threads,
+                                  grid.ctypes.data_as(C.c_void_p) if absorption else None)
     if st not in (0, _abi.ERR_CANCELLED):
         raise RuntimeError(f"oracle run failed ({st}): {lib().ymc_oracle_last_error().decode()}")
     res.status = st
     res.records = rec
+    res.absorption = grid
     return res
 
 

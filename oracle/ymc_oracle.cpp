@@ -226,6 +226,7 @@ struct Photon {
 struct Acc {
   uint64_t A_w = 0, killed_w = 0, killed = 0, steps = 0, fresnel = 0;
   int64_t roulette_w = 0;
+  uint64_t *abs = nullptr;  // absorption grid of the packet's table (optional)
 };
 
 // Launch of a new packet: photontrack_reinitialize (src/mc3d_gpu.cu:74-111) followed by the
@@ -319,6 +320,14 @@ bool step(Photon &p, uint64_t &rx, uint32_t ra, const Geom &G, const Props &P, i
   float W0 = p.W;
   p.W = p.W - dW * (float)(!hit);
   acc.A_w += fx(W0) - fx(p.W);
+  // Absorption grid (the disabled deposit of src/mc3d_gpu.cu:265-272): the absorbed weight goes
+  // to the voxel the step started in, clamped to the grid like the texture lookup.
+  if (acc.abs && !hit) {
+    const int ic = std::min(std::max(p.i, 0), G.nx - 1);
+    const int jc = std::min(std::max(p.j, 0), G.ny - 1);
+    const int kc = std::min(std::max(p.k, 0), G.nz - 1);
+    __atomic_fetch_add(&acc.abs[((size_t)kc * G.ny + jc) * G.nx + ic], fx(W0) - fx(p.W), __ATOMIC_RELAXED);
+  }
 
   // Russian roulette (:275-277); SURVIVAL_CHANCE and W_THRESHOLD are double constants.
   float eps_r = mwc_co(rx, ra);
@@ -527,9 +536,23 @@ uint64_t ymc_oracle_build_chains(const uint64_t *budgets, int32_t num_tables, in
 // chains[c].table from stream (x0[c], a0[c]).  Tallies follow photon_detector
 // (src/mc3d_photons.cu:151-171) on the tally grid; records (optional) are indexed by the global
 // packet number (chains in order, packets in order).  threads <= 0 uses the OpenMP default.
+// Same, with an optional absorption grid abs_grid [num_tables][nz][ny][nx] of fixed-point
+// absorbed weight (zeroed here).
+int32_t ymc_oracle_run_abs(const ymc_geometry *geom, const ymc_optprops *props, const ymc_params *prm,
+                           const ymc_chain *chains, const uint64_t *x0, const uint32_t *a0, uint64_t n_chains,
+                           ymc_tally *out, ymc_photon_record *records, uint64_t n_records, int32_t threads,
+                           uint64_t *abs_grid);
+
 int32_t ymc_oracle_run(const ymc_geometry *geom, const ymc_optprops *props, const ymc_params *prm,
                        const ymc_chain *chains, const uint64_t *x0, const uint32_t *a0, uint64_t n_chains,
                        ymc_tally *out, ymc_photon_record *records, uint64_t n_records, int32_t threads) {
+  return ymc_oracle_run_abs(geom, props, prm, chains, x0, a0, n_chains, out, records, n_records, threads, nullptr);
+}
+
+int32_t ymc_oracle_run_abs(const ymc_geometry *geom, const ymc_optprops *props, const ymc_params *prm,
+                           const ymc_chain *chains, const uint64_t *x0, const uint32_t *a0, uint64_t n_chains,
+                           ymc_tally *out, ymc_photon_record *records, uint64_t n_records, int32_t threads,
+                           uint64_t *abs_grid) {
   if (!geom || !props || !prm || !chains || !x0 || !a0 || !geom->tissue) {
     snprintf(g_err, sizeof g_err, "null argument");
     return YMC_ERR_ARG;
@@ -589,6 +612,7 @@ int32_t ymc_oracle_run(const ymc_geometry *geom, const ymc_optprops *props, cons
     }
   }
 
+  if (abs_grid) std::memset(abs_grid, 0, sizeof(uint64_t) * L * G.nx * G.ny * G.nz);
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -603,6 +627,7 @@ int32_t ymc_oracle_run(const ymc_geometry *geom, const ymc_optprops *props, cons
     uint64_t rx = x0[c];
     const uint32_t ra = a0[c];
     Acc acc;
+    if (abs_grid) acc.abs = abs_grid + (size_t)w * G.nx * G.ny * G.nz;
     uint64_t done = 0;
     for (uint32_t q = 0; q < chains[c].count; ++q) {
       if (cancel && *cancel) {

@@ -147,3 +147,22 @@ def test_roulette_is_unbiased(multipliers):
     net = int(t["roulette_w"]) / FX
     assert abs(net) < 5 * 3e-4 * np.sqrt(2.0 * t["photons"]), net
     assert t["A_w"] > 0
+
+
+def test_absorption_grid(multipliers):
+    """Absorption grid (the deposit src/mc3d_gpu.cu:265-272 has disabled): every absorbed weight
+    lands in a voxel, so the grid sums exactly to the totals' A_w; nothing is absorbed in ambient
+    voxels (mu_a = 0); and on the index-matched van de Hulst slab the absorbed fraction is
+    1 - Rd - Tt = 1 - 0.09739 - 0.66096 = 0.24165 of the launched weight."""
+    n = 100_000
+    s = configs.van_de_hulst_slab(photons=n)
+    r = O.run_scene(s, multipliers, absorption=True)
+    t = r.totals[0]
+    grid = r.absorption
+    assert grid.shape == (1, 14, 8, 8)
+    assert int(grid.sum()) == int(t["A_w"])
+    assert not grid[0, 10:].any()
+    assert abs(int(t["A_w"]) / FX / n - 0.24165) < 0.006
+    # absorption falls with depth in a slab lit from above
+    layers = grid[0, :10].sum(axis=(1, 2)).astype(float)
+    assert layers[0] > layers[-1]

@@ -16,7 +16,7 @@ STATUS_NAMES = {
 }
 ILLUM_UNIFORM, ILLUM_PENCIL = 0, 1
 FRESNEL_YMC3D, FRESNEL_EXACT = 0, 1
-FLAG_COUNTERS, FLAG_RECORD = 0x1, 0x2
+FLAG_COUNTERS, FLAG_RECORD, FLAG_ABSORPTION = 0x1, 0x2, 0x4
 
 
 class Geometry(C.Structure):
@@ -137,6 +137,7 @@ _SIGS = {
     "ymc_engine_run": (C.c_int, [C.c_void_p]),
     "ymc_engine_read_tally": (C.c_int, [C.c_void_p, P(Tally)]),
     "ymc_engine_read_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "ymc_engine_read_absorption": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "ymc_engine_last_kernel_ms": (C.c_int, [C.c_void_p, P(C.c_double)]),
     "ymc_engine_pool_lanes": (C.c_int, [C.c_void_p, P(C.c_int32)]),
     "ymc_engine_rng_fill": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint32), C.c_uint32, C.c_uint32,

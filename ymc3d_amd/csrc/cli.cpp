@@ -13,6 +13,8 @@
 //   --multipliers FILE   safeprimes_base32.txt-style multiplier table
 //   --fresnel exact      textbook Fresnel formula instead of the reference's
 //   --device D           HIP device
+//   --absorption         also tally absorbed weight per voxel and write <outname>_abs.bin and
+//                        <outname>_flu.bin (save_abs_map, src/mc3d_io.cu:229-255)
 // Outputs: <outname>_drs.bin (save_property layout, reflectance normalised by the packet count,
 // src/mc3d_io.cu:218-227), <outname>_drt.bin (same for transmittance) and
 // <outname>_simparam.dat (src/mc3d_main.cu:85-87).
@@ -71,6 +73,7 @@ int main(int argc, char **argv) {
   int tnx = 0, tny = 0;
   float tdx = 0, tdy = 0;
   const char *mult_path = nullptr;
+  bool absorption = false;
 
   if (std::strcmp(def_path, "-") != 0) {
     if (FILE *f = std::fopen(def_path, "r")) {
@@ -111,6 +114,7 @@ int main(int argc, char **argv) {
     } else if (a == "--multipliers") need(1), mult_path = argv[++i];
     else if (a == "--fresnel") need(1), fresnel = std::strcmp(argv[++i], "exact") ? YMC_FRESNEL_YMC3D : YMC_FRESNEL_EXACT;
     else if (a == "--device") need(1), device = std::atoi(argv[++i]);
+    else if (a == "--absorption") absorption = true;
     else {
       std::fprintf(stderr, "ymc3d: unknown option %s\n", a.c_str());
       return 1;
@@ -152,6 +156,7 @@ int main(int argc, char **argv) {
   prm.radial_x0 = prm.radial_y0 = -1;
   prm.fresnel_mode = fresnel;
   prm.kill_zero_step = 1;
+  if (absorption) prm.flags |= YMC_FLAG_ABSORPTION;
   if (ymc_engine_prepare(e, &prm) != YMC_OK) return die("prepare");
   std::fprintf(stderr, "\nPerforming MC3D simulation on the GPU (%llu packets, seed %llu).\n",
                (unsigned long long)photons, (unsigned long long)seed);
@@ -187,6 +192,28 @@ int main(int argc, char **argv) {
     return die("write _drs.bin");
   if (ymc_write_property((base + "_drt.bin").c_str(), 2, onx, ony, 1, odx, ody, 0, drt.data()) != YMC_OK)
     return die("write _drt.bin");
+  if (absorption) {
+    // Absorbed weight per launched packet in each voxel (_abs.bin) and that divided by the voxel's
+    // mu_a (_flu.bin, 0 where mu_a is 0), as save_abs_map lays them out.  save_abs_map multiplies by
+    // the packet count where a per-packet normalisation is meant; this writes per packet, like
+    // _drs.bin.
+    const size_t nv = (size_t)nx * ny * nz;
+    std::vector<uint64_t> ag(nv);
+    if (ymc_engine_read_absorption(e, ag.data(), nv) != YMC_OK) return die("read absorption");
+    std::vector<double> abs_map(nv), flu(nv);
+    double sum = 0;
+    for (size_t v = 0; v < nv; ++v) {
+      abs_map[v] = (double)ag[v] / YMC_FX_ONE / N;
+      const float m = mua[tissue[v]];
+      flu[v] = m > 0 ? abs_map[v] / m : 0.0;
+      sum += abs_map[v];
+    }
+    std::fprintf(stderr, "Absorbed fraction in geometry is %f.\n", sum);
+    if (ymc_write_property((base + "_abs.bin").c_str(), 3, nx, ny, nz, dx, dy, dz, abs_map.data()) != YMC_OK)
+      return die("write _abs.bin");
+    if (ymc_write_property((base + "_flu.bin").c_str(), 3, nx, ny, nz, dx, dy, dz, flu.data()) != YMC_OK)
+      return die("write _flu.bin");
+  }
   if (FILE *f = std::fopen((base + "_simparam.dat").c_str(), "w")) {
     std::fprintf(f, "%llu #num_photons\n", (unsigned long long)tot.photons);
     std::fclose(f);

@@ -142,6 +142,9 @@ struct KArgs {
   int fast_max;  // and at most fast_max iterations
   ymc_photon_record *records;
   unsigned long long n_records;
+  // absorption grid [tables][nz][ny][nx] (YMC_FLAG_ABSORPTION) or null
+  unsigned long long *abs_grid;
+  uint32_t nvox;
 };
 
 // Offsets into the per-table totals block (ymc_totals field order).
@@ -316,7 +319,7 @@ enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
 template <bool COUNTERS, bool LAYERED>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
                                             const RowConst *tab, unsigned long long *tot, LaneAcc &acc,
-                                            const ymc_layered *lg) {
+                                            const ymc_layered *lg, uint32_t table) {
   const float4 c0 = tab[p.t].r0;
   const float4 c1 = tab[p.t].r1;
   const float ni = c0.w;
@@ -394,6 +397,13 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
       const float W_in = W;
       W = W - c1.y * W;  // dW = (mu_a/mu_t)*W (:262-263)
       if (COUNTERS) acc.absorbed += fx(W_in) - fx(W);
+      if (a.abs_grid) {  // the disabled deposit of :265-272, at the voxel the step started in
+        const uint32_t ic = (uint32_t)min(max(p.i, 0), a.nx - 1);
+        const uint32_t jc = (uint32_t)min(max(p.j, 0), a.ny - 1);
+        const uint32_t kc = (uint32_t)min(max(p.k, 0), a.nz - 1);
+        atomicAdd(&a.abs_grid[(size_t)table * a.nvox + (kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic],
+                  (unsigned long long)(fx(W_in) - fx(W)));
+      }
       if (W < a.w_thresh && mu_t != 0.0f) {  // roulette (:275-277)
         const float e = u32_co(u_roul);
         const float Wr = ((double)e < YMC_SURVIVAL_CHANCE) ? (float)((double)W / YMC_SURVIVAL_CHANCE) : 0.0f;
@@ -639,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
     if (alive) {
       const RowConst *tab = rows + (size_t)table * a.num_types;
       unsigned long long *tt = totals + (size_t)table * kTotFields;
-      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, a.layered + table);
+      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, a.layered + table, table);
       if (res != kLive) {
         tally_packet<RADIAL>(p, a, table, tt);
         if (RECORD) {
@@ -761,7 +771,8 @@ struct ymc_engine {
   ymc_params params{};
   bool prepared = false;
   // tallies
-  ymc::DevBuf map_w, map_w2, map_n, Rr_w, Rr_w2, Rr_n, tot, counter, records;
+  ymc::DevBuf map_w, map_w2, map_n, Rr_w, Rr_w2, Rr_n, tot, counter, records, absorb;
+  bool absorbing = false;  // the prepared run tallies the absorption grid
   int tnx = 0, tny = 0, nr = 0;
   float tdx = 0, tdy = 0;
   uint64_t n_records = 0;
@@ -785,6 +796,9 @@ static float smallest_float_ge(double v) {
   if ((double)f < v) f = std::nextafter(f, INFINITY);
   return f;
 }
+
+// Largest absorption grid an engine allocates (tables x voxels x 8 bytes).
+constexpr size_t kMaxAbsorptionBytes = size_t(32) << 30;
 
 static ymc_status check_engine(ymc_engine *e) {
   if (!e) return fail(YMC_ERR_ARG, "null engine");
@@ -1015,6 +1029,13 @@ static ymc_status upload_chains(ymc_engine *e, const ymc_chain *chains, const ui
   } else {
     e->n_records = 0;
   }
+  e->absorbing = (e->params.flags & YMC_FLAG_ABSORPTION) != 0;
+  if (e->absorbing) {
+    const size_t bytes = (size_t)e->num_tables * e->nx * e->ny * e->nz * 8;
+    if (bytes > kMaxAbsorptionBytes)
+      return fail(YMC_ERR_ARG, "absorption grid of %zu bytes exceeds the %zu-byte limit", bytes, kMaxAbsorptionBytes);
+    if ((st = e->absorb.ensure(bytes))) return st;
+  }
   e->n_chains = n;
   e->n_packets = total;
   e->prepared = true;
@@ -1068,6 +1089,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   HIP_TRY(hipMemsetAsync(e->tot.p, 0, L * sizeof(ymc_totals), s));
   HIP_TRY(hipMemsetAsync(e->counter.p, 0, 8, s));
   if (e->n_records) HIP_TRY(hipMemsetAsync(e->records.p, 0, e->n_records * sizeof(ymc_photon_record), s));
+  const size_t nvox = (size_t)e->nx * e->ny * e->nz;
+  if (e->absorbing) HIP_TRY(hipMemsetAsync(e->absorb.p, 0, L * nvox * 8, s));
 
   KArgs a{};
   a.vox = e->vox.as<uint16_t>();
@@ -1120,6 +1143,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   a.fast_max = prm.fast_max_iters > 0 ? prm.fast_max_iters : 64;
   a.records = e->records.as<ymc_photon_record>();
   a.n_records = e->n_records;
+  a.abs_grid = e->absorbing ? e->absorb.as<unsigned long long>() : nullptr;
+  a.nvox = (uint32_t)nvox;
 
   const bool uniform = prm.illumination == YMC_ILLUM_UNIFORM;
   const bool counters = (prm.flags & YMC_FLAG_COUNTERS) != 0;
@@ -1174,6 +1199,19 @@ ymc_status ymc_engine_read_tally(ymc_engine *e, ymc_tally *out) {
   HIP_TRY(cp(out->Rr_w2, e->Rr_w2, L * e->nr * 8));
   HIP_TRY(cp(out->Rr_n, e->Rr_n, L * e->nr * 4));
   HIP_TRY(cp(out->totals, e->tot, L * sizeof(ymc_totals)));
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_read_absorption(ymc_engine *e, uint64_t *out, uint64_t count) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!out) return fail(YMC_ERR_ARG, "null output");
+  if (!e->prepared || !e->absorbing) return fail(YMC_ERR_STATE, "last run did not tally absorption (YMC_FLAG_ABSORPTION)");
+  const uint64_t n = (uint64_t)e->num_tables * e->nx * e->ny * e->nz;
+  if (count != n) return fail(YMC_ERR_ARG, "absorption grid has %llu entries, not %llu", (unsigned long long)n,
+                              (unsigned long long)count);
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipMemcpy(out, e->absorb.p, n * 8, hipMemcpyDeviceToHost));
   return YMC_OK;
 }
 

@@ -303,6 +303,15 @@ class Engine:
               "ymc_engine_read_records")
         return out
 
+    def read_absorption(self) -> np.ndarray:
+        """Absorption grid of the last run made with FLAG_ABSORPTION: fixed-point absorbed weight
+        (raw / 2^32) per voxel, [tables, nz, ny, nx]."""
+        nx, ny, nz = self._shape[:3]
+        out = np.zeros((self._tables, nz, ny, nx), dtype=np.uint64)
+        check(self._lib.ymc_engine_read_absorption(self._h, out.ctypes.data_as(C.c_void_p), out.size),
+              "ymc_engine_read_absorption")
+        return out
+
     def simulate(self, **params) -> TallyResult:
         self.prepare(**params)
         self.run()
