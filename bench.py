@@ -33,13 +33,15 @@ PEAK_FP32_TFLOPS = 157.3
 # script's default workload, scaled by the same counters read on a 1 GiB device copy (x2.000 for
 # FETCH_SIZE, x1.000 for WRITE_SIZE); profiles/transport_skin_vessels_256x256x128.md.  The launch
 # is deterministic (same chains and streams every step), so the figure holds for every default run.
-MEASURED_TRAFFIC = {("skin_vessels", 100_000_000, 458752): 1.3968e12}
+MEASURED_TRAFFIC = {("skin_vessels", 100_000_000, 458752): 1.4076e12}
 
 WORKLOADS = {
     "skin_vessels": ("skin_vessels", 100_000_000),
     "skin_slab_sphere": ("skin_slab_sphere", 1_000_000),
     "batched_spectra": ("batched_spectra", 0),
+    "surrogate": ("surrogate", 0),
 }
+PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
 
 
 def main() -> int:
@@ -70,6 +72,8 @@ def main() -> int:
     import ymc3d_amd as y
     from ymc3d_amd import configs
 
+    if args.workload == "surrogate":
+        return surrogate_bench(args)
     fn, default_n = WORKLOADS[args.workload]
     if fn == "batched_spectra":
         scene = configs.batched_spectra()
@@ -157,6 +161,79 @@ def main() -> int:
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, tab, params, n_photons, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def surrogate_bench(args) -> int:
+    """Reflectance-surrogate serving: a 4-member ensemble of 12-256-256-256-33 GELU networks
+    (random initialisation), batches of 4096 parameter vectors resident on the device, one pass per
+    member per batch; a spectrum is 64 wavelengths.  Also one training epoch (Adam, batch 512) over
+    65536 synthetic samples.  Roofline: BF16 MFMA FLOPs of the products over the kernel time."""
+    import numpy as np
+
+    from ymc3d_amd import surrogate as S
+
+    members, rows, reps = 4, 4096, max(50, args.steps * 50)
+    nets = [S.MLP(seed=m) for m in range(members)]
+    flops_row = 2 * sum(a * b for a, b in zip(S.WIDTHS[:-1], S.WIDTHS[1:]))
+    for _ in range(args.warmup):
+        for n in nets:
+            n.time_forward(rows, 10)
+    t0 = time.perf_counter()
+    ms = [n.time_forward(rows, reps) for n in nets]
+    wall = time.perf_counter() - t0
+    ms_pass = float(np.mean(ms))
+    queries_per_s = rows / (members * ms_pass * 1e-3)
+    achieved = flops_row * rows / (ms_pass * 1e-3) / 1e12
+    # training epoch
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (65536, S.WIDTHS[0])).astype(np.float32)
+    yv = rng.normal(size=(65536, S.WIDTHS[-1])).astype(np.float32)
+    nets[0].fit(x[:4096], yv[:4096], 1, 512)
+    t0 = time.perf_counter()
+    nets[0].fit(x, yv, 1, 512)
+    t_train = time.perf_counter() - t0
+    for n in nets:
+        n.close()
+    out = {
+        "metric": "surrogate reflectance spectra/s (1 GPU)",
+        "value": queries_per_s / 64,
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_pass * members,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "surrogate_ensemble_4x_12-256-256-256-33", "rows_per_pass": rows,
+                   "members": members, "queries_per_s": queries_per_s, "ms_per_member_pass": ms_pass,
+                   "timed_wall_s": wall, "training_samples_per_s": 65536 / t_train,
+                   "training_tflops": 3 * flops_row * 65536 / t_train / 1e12},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import mlp_oracle as R
+
+        p = S.MLP(seed=0)
+        params = p.get_params().astype(np.float64)
+        p.close()
+        xq = rng.uniform(-1, 1, (4096, S.WIDTHS[0]))
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            R.forward(params, S.WIDTHS, xq)
+            done += 4096
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / members / dt / 64, "unit": "spectra/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                               "kind": "port", "sample": f"{done} rows through one float64 numpy network, {dt:.1f} s, "
+                                                         f"scaled to {members} members"}
     print(json.dumps(out), flush=True)
     return 0
 

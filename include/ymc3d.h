@@ -278,6 +278,15 @@ ymc_status ymc_run_3dmc_gpu(int32_t nx, int32_t ny, int32_t nz, float dx, float 
                             double *R_tot, double *T, double *T_tot, float *A,
                             int32_t *num_finished);
 
+/* One-shot run through a temporary engine on `device`: geometry, optical-property tables (one or
+ * many, with per-table budgets in params->budgets), parameters and seed as for the engine API;
+ * tallies into `out` (caller-sized: tables x tally grid, tables x radial bins, tables totals) and,
+ * with YMC_FLAG_ABSORPTION, the absorption grid into `absorption` (tables x nz x ny x nx).  The
+ * whole of run_3dmc_gpu (src/mc3d_gpu.cu:388-601) with every parameter explicit; returns
+ * YMC_ERR_CANCELLED with the finished packets tallied when params->cancel is raised. */
+ymc_status ymc_run(int32_t device, const uint32_t *table, uint32_t table_len, const ymc_geometry *geom,
+                   const ymc_optprops *props, const ymc_params *params, ymc_tally *out, uint64_t *absorption);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Reflectance surrogate: MLP with GELU hidden layers trained and served on the device with BF16
  * matrix-core products (FP32 accumulation) and Adam on FP32 master weights.  No counterpart in the
@@ -294,6 +303,9 @@ ymc_status ymc_mlp_set_params(ymc_mlp *n, const float *params, uint64_t count); 
 ymc_status ymc_mlp_get_params(ymc_mlp *n, float *params, uint64_t count);
 /* Outputs y [rows x out] for inputs x [rows x in], in launches of 4096 rows. */
 ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y);
+/* Timing hook: device milliseconds per forward pass of `rows` rows already resident on the device
+ * (zero inputs), averaged over `reps` passes timed with HIP events on the network's stream. */
+ymc_status ymc_mlp_time_forward(ymc_mlp *n, int32_t rows, int32_t reps, double *ms_per_pass);
 /* Mean-squared-error loss and its gradient (flat, like the parameters) for one batch (test hook). */
 ymc_status ymc_mlp_gradients(ymc_mlp *n, const float *x, const float *y, int32_t rows, float *grad, float *loss);
 /* Uploads a training set (x [rows x in], y [rows x out]). */

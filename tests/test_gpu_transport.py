@@ -229,3 +229,27 @@ def test_cli_end_to_end(tmp_path, multipliers):
     assert drs.shape == (64, 64) and d == pytest.approx((0.01, 0.01))
     assert (tmp_path / "run_simparam.dat").read_text().startswith("50000 #num_photons")
     assert 0.1 < drs.sum() < 0.3
+
+
+def test_one_shot_run_matches_engine(engine, multipliers):
+    """ymc_run (one temporary engine, every parameter explicit) equals the engine API bit for bit,
+    absorption grid included, for a batched two-table run with uneven budgets."""
+    base = configs.skin_slab_sphere(photons=1)
+    mua, mus, g, n = base.optprops()
+    props = tuple(np.concatenate([v, v * np.float32(1.5) if q == 0 else v]) for q, v in enumerate((mua, mus, g, n)))
+    budgets = np.array([30_000, 7_000], dtype=np.uint64)
+    params = {k: v for k, v in base.params.items() if k != "num_photons"}
+    params.update(budgets=budgets, flags=y.FLAG_COUNTERS | y.FLAG_ABSORPTION)
+    engine.set_geometry(base.rows, *base.voxel)
+    engine.set_optprops(*props)
+    engine.prepare(**params)
+    engine.run()
+    want = engine.read_tally()
+    want_abs = engine.read_absorption()
+    got = y.run_once(base.rows, base.voxel, props, multipliers, **params)
+    assert got.status == 0
+    for f in ("R_w", "R_w2", "R_n", "T_w", "T_w2", "T_n"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f
+    assert got.totals.tobytes() == want.totals.tobytes()
+    assert np.array_equal(got.absorption, want_abs)
+    assert list(got.totals["photons"]) == [30_000, 7_000]

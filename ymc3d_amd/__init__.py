@@ -7,8 +7,8 @@ of the reference's driver: file formats, scene generators and the engine wrapper
 """
 from ._abi import FX_ONE, FRESNEL_EXACT, FRESNEL_YMC3D, ILLUM_PENCIL, ILLUM_UNIFORM, FLAG_COUNTERS, FLAG_RECORD, FLAG_ABSORPTION
 from .engine import (Engine, TallyResult, YmcError, build_chains, default_multipliers, generate_multipliers,
-                     layered_voxelize, load_library, read_multipliers_text, run_3dmc_gpu, seed_streams)
+                     layered_voxelize, load_library, read_multipliers_text, run_3dmc_gpu, run_once, seed_streams)
 
 __all__ = ["Engine", "TallyResult", "YmcError", "build_chains", "default_multipliers", "generate_multipliers",
-           "layered_voxelize", "load_library", "read_multipliers_text", "run_3dmc_gpu", "seed_streams", "FX_ONE", "FRESNEL_EXACT",
+           "layered_voxelize", "load_library", "read_multipliers_text", "run_3dmc_gpu", "run_once", "seed_streams", "FX_ONE", "FRESNEL_EXACT",
            "FRESNEL_YMC3D", "ILLUM_PENCIL", "ILLUM_UNIFORM", "FLAG_COUNTERS", "FLAG_RECORD", "FLAG_ABSORPTION"]

@@ -147,12 +147,15 @@ _SIGS = {
                                    P(C.c_float), C.c_int32, C.c_uint64, P(C.c_uint32), C.c_uint32,
                                    P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_float),
                                    P(C.c_int32)]),
+    "ymc_run": (C.c_int, [C.c_int32, P(C.c_uint32), C.c_uint32, P(Geometry), P(OptProps), P(Params), P(Tally),
+                          C.c_void_p]),
     "ymc_mlp_create": (C.c_int, [C.c_int32, P(C.c_int32), C.c_int32, C.c_uint64, P(C.c_void_p)]),
     "ymc_mlp_destroy": (None, [C.c_void_p]),
     "ymc_mlp_num_params": (C.c_uint64, [C.c_void_p]),
     "ymc_mlp_set_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_uint64]),
     "ymc_mlp_get_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_uint64]),
     "ymc_mlp_forward": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float)]),
+    "ymc_mlp_time_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, P(C.c_double)]),
     "ymc_mlp_gradients": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float), P(C.c_float)]),
     "ymc_mlp_set_data": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64]),
     "ymc_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, P(C.c_float)]),

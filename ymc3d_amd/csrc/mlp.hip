@@ -428,6 +428,29 @@ ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y) {
   return YMC_OK;
 }
 
+ymc_status ymc_mlp_time_forward(ymc_mlp *n, int32_t rows, int32_t reps, double *ms_per_pass) {
+  if (!n || !ms_per_pass || rows < 1 || reps < 1) return fail(YMC_ERR_ARG, "ymc_mlp_time_forward: bad arguments");
+  MLP_TRY(hipSetDevice(n->device));
+  ymc_status st;
+  if ((st = ensure_capacity(n, rows))) return st;
+  MLP_TRY(hipMemsetAsync(n->H[0], 0, (size_t)rows * n->widths[0] * sizeof(uint16_t), n->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  MLP_TRY(hipEventCreate(&e0));
+  MLP_TRY(hipEventCreate(&e1));
+  forward(n, rows);  // warm-up
+  MLP_TRY(hipEventRecord(e0, n->stream));
+  for (int r = 0; r < reps; ++r) forward(n, rows);
+  MLP_TRY(hipEventRecord(e1, n->stream));
+  MLP_TRY(hipGetLastError());
+  MLP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  MLP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_pass = (double)ms / reps;
+  return YMC_OK;
+}
+
 ymc_status ymc_mlp_gradients(ymc_mlp *n, const float *x, const float *y, int32_t rows, float *grad, float *loss) {
   if (!n || !x || !y || !grad || !loss || rows < 1) return fail(YMC_ERR_ARG, "ymc_mlp_gradients: bad arguments");
   MLP_TRY(hipSetDevice(n->device));

@@ -1307,4 +1307,31 @@ ymc_status ymc_run_3dmc_gpu(int32_t nx, int32_t ny, int32_t nz, float dx, float 
   return YMC_OK;
 }
 
+ymc_status ymc_run(int32_t device, const uint32_t *table, uint32_t table_len, const ymc_geometry *geom,
+                   const ymc_optprops *props, const ymc_params *params, ymc_tally *out, uint64_t *absorption) {
+  if (!geom || !props || !params || !out) return fail(YMC_ERR_ARG, "ymc_run: null argument");
+  if (((params->flags & YMC_FLAG_ABSORPTION) != 0) != (absorption != nullptr))
+    return fail(YMC_ERR_ARG, "ymc_run: pass an absorption buffer exactly when YMC_FLAG_ABSORPTION is set");
+  if (params->flags & YMC_FLAG_RECORD) return fail(YMC_ERR_ARG, "ymc_run: records need the engine API");
+  ymc_engine *e = nullptr;
+  ymc_status st = ymc_engine_create(device, table, table_len, &e);
+  if (st) return st;
+  struct Guard {
+    ymc_engine *e;
+    ~Guard() { ymc_engine_destroy(e); }
+  } guard{e};
+  if ((st = ymc_engine_set_geometry(e, geom)) || (st = ymc_engine_set_optprops(e, props)) ||
+      (st = ymc_engine_prepare(e, params)))
+    return st;
+  st = ymc_engine_run(e);
+  if (st != YMC_OK && st != YMC_ERR_CANCELLED) return st;
+  ymc_status rs = ymc_engine_read_tally(e, out);
+  if (rs) return rs;
+  if (absorption) {
+    const uint64_t n = (uint64_t)props->num_tables * geom->nx * geom->ny * geom->nz;
+    if ((rs = ymc_engine_read_absorption(e, absorption, n))) return rs;
+  }
+  return st;
+}
+
 }  // extern "C"

@@ -356,3 +356,55 @@ def run_3dmc_gpu(tissue_rows: np.ndarray, dx: float, dy: float, dz: float, mua, 
                                ptr(R, C.c_double), C.byref(rt), ptr(T, C.c_double), C.byref(tt), ptr(A, C.c_float),
                                C.byref(nf)), "ymc_run_3dmc_gpu")
     return R, rt.value, T, tt.value, nf.value
+
+
+def run_once(tissue_rows: np.ndarray, voxel, optprops, multipliers: np.ndarray | None = None, device: int = 0,
+             **params) -> TallyResult:
+    """One-shot run (ymc_run): uint8 remapped rows [nz, ny, nx], voxel sizes, optical properties
+    (mua, mus, g, n as [tables, T] or [T]) and ymc_params fields as keywords (budgets for several
+    tables).  With flags including FLAG_ABSORPTION the result carries `.absorption`
+    [tables, nz, ny, nx]."""
+    lib = load_library()
+    rows = np.ascontiguousarray(tissue_rows, dtype=np.uint8)
+    nz, ny, nx = rows.shape
+    arrs = []
+    for v in optprops:
+        v = np.asarray(v, dtype=np.float32)
+        arrs.append(np.ascontiguousarray(v[None] if v.ndim == 1 else v))
+    L, T = arrs[0].shape
+    geo = _abi.Geometry(nx, ny, nz, voxel[0], voxel[1], voxel[2], ptr(rows, C.c_uint8))
+    op = _abi.OptProps(T, L, *(ptr(v, C.c_float) for v in arrs))
+    kw = dict(params)
+    budgets = kw.pop("budgets", None)
+    cancel = kw.pop("cancel", None)
+    p = _abi.default_params(**kw)
+    keep = []
+    if budgets is not None:
+        b = np.ascontiguousarray(budgets, dtype=np.uint64)
+        if b.size != L:
+            raise ValueError("budgets must have one entry per table")
+        p.budgets = ptr(b, C.c_uint64)
+        keep.append(b)
+    if cancel is not None:
+        p.cancel = C.cast(cancel, C.POINTER(C.c_int32))
+    tnx = p.tally_nx if p.tally_nx > 0 else nx
+    tny = p.tally_ny if p.tally_ny > 0 else ny
+    nr = max(p.radial_bins, 0)
+    mk = lambda dt, *s: np.zeros(s, dtype=dt)
+    res = TallyResult(mk(np.uint64, L, tny, tnx), mk(np.uint64, L, tny, tnx), mk(np.uint32, L, tny, tnx),
+                      mk(np.uint64, L, tny, tnx), mk(np.uint64, L, tny, tnx), mk(np.uint32, L, tny, tnx),
+                      mk(np.uint64, L, nr), mk(np.uint64, L, nr), mk(np.uint32, L, nr),
+                      np.zeros(L, dtype=_abi.TOTALS_DTYPE))
+    t = _abi.Tally(ptr(res.R_w, C.c_uint64), ptr(res.R_w2, C.c_uint64), ptr(res.R_n, C.c_uint32),
+                   ptr(res.T_w, C.c_uint64), ptr(res.T_w2, C.c_uint64), ptr(res.T_n, C.c_uint32),
+                   ptr(res.Rr_w, C.c_uint64), ptr(res.Rr_w2, C.c_uint64), ptr(res.Rr_n, C.c_uint32),
+                   res.totals.ctypes.data_as(C.c_void_p))
+    grid = np.zeros((L, nz, ny, nx), dtype=np.uint64) if p.flags & _abi.FLAG_ABSORPTION else None
+    tab = np.ascontiguousarray(default_multipliers() if multipliers is None else multipliers, dtype=np.uint32)
+    st = lib.ymc_run(device, ptr(tab, C.c_uint32), tab.size, C.byref(geo), C.byref(op), C.byref(p), C.byref(t),
+                     grid.ctypes.data_as(C.c_void_p) if grid is not None else None)
+    if st not in (_abi.OK, _abi.ERR_CANCELLED):
+        check(st, "ymc_run")
+    res.status = st
+    res.absorption = grid
+    return res

@@ -57,6 +57,12 @@ class MLP:
         check(self._lib.ymc_mlp_forward(self._h, ptr(x, C.c_float), x.shape[0], ptr(y, C.c_float)), "ymc_mlp_forward")
         return y
 
+    def time_forward(self, rows: int, reps: int) -> float:
+        """Device milliseconds per forward pass of `rows` device-resident rows."""
+        ms = C.c_double()
+        check(self._lib.ymc_mlp_time_forward(self._h, rows, reps, C.byref(ms)), "ymc_mlp_time_forward")
+        return ms.value
+
     def gradients(self, x: np.ndarray, y: np.ndarray) -> tuple[float, np.ndarray]:
         x = np.ascontiguousarray(x, np.float32)
         y = np.ascontiguousarray(y, np.float32)
