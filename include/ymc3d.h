@@ -303,9 +303,13 @@ ymc_status ymc_mlp_set_params(ymc_mlp *n, const float *params, uint64_t count); 
 ymc_status ymc_mlp_get_params(ymc_mlp *n, float *params, uint64_t count);
 /* Outputs y [rows x out] for inputs x [rows x in], in launches of 4096 rows. */
 ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y);
-/* Timing hook: device milliseconds per forward pass of `rows` rows already resident on the device
- * (zero inputs), averaged over `reps` passes timed with HIP events on the network's stream. */
-ymc_status ymc_mlp_time_forward(ymc_mlp *n, int32_t rows, int32_t reps, double *ms_per_pass);
+/* The same inputs through `count` (1..8) networks of identical widths (each <= 256, <= 8 layers) on
+ * one device, in one fused launch per 4096 rows: y [count][rows][out]. */
+ymc_status ymc_mlp_forward_ensemble(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y);
+/* Timing hook: device milliseconds per fused ensemble pass of `rows` (<= 4096) rows already resident
+ * on the device (zero inputs), averaged over `reps` launches timed with HIP events on the first
+ * network's stream. */
+ymc_status ymc_mlp_time_forward(ymc_mlp *const *nets, int32_t count, int32_t rows, int32_t reps, double *ms_per_pass);
 /* Mean-squared-error loss and its gradient (flat, like the parameters) for one batch (test hook). */
 ymc_status ymc_mlp_gradients(ymc_mlp *n, const float *x, const float *y, int32_t rows, float *grad, float *loss);
 /* Uploads a training set (x [rows x in], y [rows x out]). */

@@ -67,3 +67,26 @@ def test_training_reduces_loss_and_fits_a_smooth_map():
     print(f"training MSE {before:.4f} -> {after:.6f}; last-batch losses {losses[0]:.5f} -> {losses[-1]:.6f}")
     assert after < 0.02 * before and after < 2e-3
     assert losses[-1] < losses[0]
+
+
+def test_fused_ensemble_forward():
+    """The fused ensemble pass (every layer in one launch, 32x32x16 BF16 products, rows chunked by
+    4096) against the FP32 reference for each member, against each member alone, and against the
+    layer-by-layer products of the training path (same loss on a batch within 0.2%)."""
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (5000, 12)).astype(np.float32)
+    y = rng.normal(size=(5000, 33)).astype(np.float32)
+    nets = [S.MLP(seed=s) for s in (11, 12, 13)]
+    try:
+        ens = S.forward_ensemble(nets, x)
+        assert ens.shape == (3, 5000, 33)
+        for q, n in enumerate(nets):
+            np.testing.assert_array_equal(ens[q], n.forward(x))
+            want, _, _ = R.forward(_bf16(n.get_params()).astype(np.float64), S.WIDTHS, _bf16(x))
+            rel = np.linalg.norm(ens[q] - want) / np.linalg.norm(want)
+            assert rel < 0.03, rel
+            loss, _ = n.gradients(x[:512], y[:512])
+            assert float(np.mean((ens[q][:512] - y[:512]) ** 2)) == pytest.approx(loss, rel=2e-3)
+    finally:
+        for n in nets:
+            n.close()

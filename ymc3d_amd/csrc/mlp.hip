@@ -54,16 +54,19 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16(const uint16_t *__restrict
                                                       const uint16_t *__restrict__ B, int ldb, int transB, int M,
                                                       int N, int K, const float *__restrict__ bias,
                                                       float *__restrict__ outF, int ldo, uint16_t *__restrict__ outH,
-                                                      int ldh) {
+                                                      int ldh, int kchunk) {
   __shared__ uint16_t sA[kTile][kSlice + 4];  // [m][k]
   __shared__ uint16_t sB[kTile][kSlice + 4];  // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  // split K: workgroup z sums k in [kb, ke) and writes its own M x ldo slab of outF
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  outF += (size_t)blockIdx.z * M * ldo;
   float16_t acc;
   for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
 
-  for (int k0 = 0; k0 < K; k0 += kSlice) {
+  for (int k0 = kb; k0 < ke; k0 += kSlice) {
     // stage 64x32 of op(A) and 32x64 of op(B): 2048 elements each, 8 per thread
     for (int e = tid; e < kTile * kSlice; e += kThreads) {
       int m, k;
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16(const uint16_t *__restrict
       const int gm = m0 + m;
       int gk = k0 + k;
       uint16_t v = 0;
-      if (gm < M && gk < K) v = transA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+      if (gm < M && gk < ke) v = transA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
       sA[m][k] = v;
       int n;
       if (transB) {  // consecutive threads walk k
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16(const uint16_t *__restrict
       const int gn = n0 + n;
       gk = k0 + k;
       uint16_t w = 0;
-      if (gn < N && gk < K) w = transB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+      if (gn < N && gk < ke) w = transB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
       sB[n][k] = w;
     }
     __syncthreads();
@@ -123,6 +126,15 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16(const uint16_t *__restrict
   }
 }
 
+// out[i] = sum over s < parts of ws[s][i], in order (split-K partial products).
+__global__ void split_reduce(const float *ws, int parts, int n, float *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float t = 0.0f;
+  for (int q = 0; q < parts; ++q) t += ws[(size_t)q * n + i];
+  out[i] = t;
+}
+
 // dZ = dA * gelu'(Z) (FP32), rounded to BF16 for the next products.
 __global__ void gelu_backward(const float *dA, const float *Z, float *dZ, uint16_t *dZh, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,13 +165,23 @@ __global__ void mse_grad(const float *out, const float *y, float *dOut, uint16_t
   if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0];
 }
 
-// db[c] = sum over rows of dZ[r][c] (FP32).
+// db[c] = sum over rows of dZ[r][c] (FP32).  A workgroup of 256 threads sums 32 columns: 8 row
+// groups per column (coalesced reads of 32 consecutive columns), then a fixed-order reduction of
+// the 8 partial sums, so the result does not depend on scheduling.
 __global__ void column_sum(const float *dZ, int rows, int cols, float *db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+  __shared__ float part[8][32];
+  const int cl = threadIdx.x & 31, rg = threadIdx.x >> 5;
+  const int c = blockIdx.x * 32 + cl;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s += dZ[(size_t)r * cols + c];
-  db[c] = s;
+  if (c < cols)
+    for (int r = rg; r < rows; r += 8) s += dZ[(size_t)r * cols + c];
+  part[rg][cl] = s;
+  __syncthreads();
+  if (rg == 0 && c < cols) {
+    float t = 0.0f;
+    for (int q = 0; q < 8; ++q) t += part[q][cl];
+    db[c] = t;
+  }
 }
 
 // Adam on FP32 master parameters, refreshing the BF16 weight copy.
@@ -190,6 +212,101 @@ __global__ void gather_rows(const float *X, const float *Y, const int *idx, int 
   const size_t s = (size_t)idx[r];
   for (int c = t; c < cx; c += blockDim.x) xb[(size_t)r * cx + c] = f2bf(X[s * cx + c]);
   for (int c = t; c < cy; c += blockDim.x) yb[(size_t)r * cy + c] = Y[s * cy + c];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused ensemble forward pass: one workgroup carries kFusedRows rows of the input through every
+// layer of one network (blockIdx.y selects the network), the activations staying in LDS as BF16;
+// v_mfma_f32_32x32x16_bf16 products with FP32 accumulation read A (activations) from LDS and B
+// from a transposed, zero-padded BF16 weight copy wt_l [Npad_l][K16_l] (K rounded up to 16, N to
+// 32), so both operands are 16-byte loads.  Widths up to kFusedMax.
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int kFusedRows = 64, kFusedMax = 256, kFusedPitch = kFusedMax + 8, kFusedLayers = 8, kFusedNets = 8;
+
+struct FusedLayer {
+  const uint16_t *wt;  // [npad][k16] BF16
+  const float *bias;   // [n]
+  int k16, n, npad;
+};
+struct FusedNet {
+  FusedLayer layer[kFusedLayers];
+  float *y;  // [rows][out] FP32
+};
+struct FusedArgs {
+  FusedNet net[kFusedNets];
+  const float *x;  // [rows][in] FP32, shared by all networks
+  int rows, in, L;
+};
+
+__global__ __launch_bounds__(256) void mlp_forward_fused(FusedArgs args) {
+  __shared__ __align__(16) uint16_t act[kFusedRows][kFusedPitch];
+  const FusedNet &net = args.net[blockIdx.y];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * kFusedRows;
+  const int k16_0 = net.layer[0].k16;
+  for (int e = tid; e < kFusedRows * k16_0; e += 256) {
+    const int m = e / k16_0, k = e % k16_0;
+    float v = 0.0f;
+    if (r0 + m < args.rows && k < args.in) v = args.x[(size_t)(r0 + m) * args.in + k];
+    act[m][k] = f2bf(v);
+  }
+  __syncthreads();
+  // wave w takes row tile w/2 and half of the 32-column tiles
+  const int rt = wave >> 1, half = wave & 1;
+  const int lr = lane & 31, lk = (lane >> 5) * 8;
+  for (int l = 0; l < args.L; ++l) {
+    const FusedLayer ly = net.layer[l];
+    const int ctiles = ly.npad / 32;
+    const int cpw = (ctiles + 1) >> 1;
+    const int c0 = half * cpw;
+    const int cn = min(cpw, ctiles - c0);
+    float16_t acc[4];
+    for (int j = 0; j < 4; ++j)
+      for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
+    for (int k0 = 0; k0 < ly.k16; k0 += 16) {
+      const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&act[rt * 32 + lr][k0 + lk]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j < cn) {
+          const bf16x8_t b =
+              *reinterpret_cast<const bf16x8_t *>(ly.wt + (size_t)((c0 + j) * 32 + lr) * ly.k16 + k0 + lk);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();  // every wave has read this layer's input
+    const bool last = l + 1 == args.L;
+    const int next_k16 = last ? 0 : net.layer[l + 1].k16;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= cn) continue;
+      const int col = (c0 + j) * 32 + lr;
+      for (int v = 0; v < 16; ++v) {
+        const int row = rt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (col < ly.n) {
+          const float z = acc[j][v] + ly.bias[col];
+          if (last) {
+            if (r0 + row < args.rows) net.y[(size_t)(r0 + row) * ly.n + col] = z;
+          } else {
+            act[row][col] = f2bf(gelu(z));
+          }
+        } else if (!last && col < next_k16) {
+          act[row][col] = 0;  // zero padding of the next layer's K
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// wt [npad][k16] from the row-major BF16 weights W [k][n], zero outside k < K, n < N.
+__global__ void transpose_pad(const uint16_t *W, int K, int N, uint16_t *wt, int k16, int npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad * k16) return;
+  const int n = i / k16, k = i % k16;
+  wt[i] = (n < N && k < K) ? W[(size_t)k * N + n] : (uint16_t)0;
 }
 
 }  // namespace mlp
@@ -226,6 +343,14 @@ struct ymc_mlp {
   int64_t n_data = 0;
   int *idx = nullptr;
   int64_t adam_t = 0;
+  // transposed, padded BF16 weights for the fused forward pass; rebuilt when the weights change
+  uint16_t *wt = nullptr;
+  std::vector<size_t> wt_off;
+  std::vector<int> k16, npad;
+  bool wt_dirty = true;
+  float *ws = nullptr;  // split-K partial weight gradients [kSplits][max in*out]
+  float *xf = nullptr;  // fused-pass input staging [4096 x in]
+  int xf_rows = 0;
 };
 
 namespace {
@@ -268,17 +393,36 @@ ymc_status ensure_capacity(ymc_mlp *n, int batch) {
 }
 
 void gemm(hipStream_t s, int epi, const uint16_t *A, int lda, int tA, const uint16_t *B, int ldb, int tB, int M, int N,
-          int K, const float *bias, float *outF, int ldo, uint16_t *outH, int ldh) {
-  dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
+          int K, const float *bias, float *outF, int ldo, uint16_t *outH, int ldh, int splits = 1) {
+  const int kchunk = splits > 1 ? ((K + splits - 1) / splits + kSlice - 1) / kSlice * kSlice : K;
+  dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, splits);
   if (epi == kStoreF32)
     hipLaunchKernelGGL(gemm_bf16<kStoreF32>, grid, dim3(kThreads), 0, s, A, lda, tA, B, ldb, tB, M, N, K, bias, outF,
-                       ldo, outH, ldh);
+                       ldo, outH, ldh, kchunk);
   else if (epi == kBiasGelu)
     hipLaunchKernelGGL(gemm_bf16<kBiasGelu>, grid, dim3(kThreads), 0, s, A, lda, tA, B, ldb, tB, M, N, K, bias, outF,
-                       ldo, outH, ldh);
+                       ldo, outH, ldh, kchunk);
   else
     hipLaunchKernelGGL(gemm_bf16<kBias>, grid, dim3(kThreads), 0, s, A, lda, tA, B, ldb, tB, M, N, K, bias, outF, ldo,
-                       outH, ldh);
+                       outH, ldh, kchunk);
+}
+
+// Weight gradient dW [M x N] = X^T dZ over `batch` rows: split over K (the batch) into up to
+// kSplits partial products reduced in a fixed order when the output has few 64x64 tiles.
+constexpr int kSplits = 8;
+void weight_grad(ymc_mlp *n, const uint16_t *X, int M, const uint16_t *dZ, int N, int batch, float *dW);
+
+void weight_grad(ymc_mlp *n, const uint16_t *X, int M, const uint16_t *dZ, int N, int batch, float *dW) {
+  const int tiles = ((M + kTile - 1) / kTile) * ((N + kTile - 1) / kTile);
+  int splits = std::min(kSplits, std::max(1, 128 / tiles));
+  splits = std::min(splits, std::max(1, batch / (2 * kSlice)));
+  if (splits <= 1) {
+    gemm(n->stream, kStoreF32, X, M, 1, dZ, N, 0, M, N, batch, nullptr, dW, N, nullptr, 0);
+    return;
+  }
+  gemm(n->stream, kStoreF32, X, M, 1, dZ, N, 0, M, N, batch, nullptr, n->ws, N, nullptr, 0, splits);
+  const int cnt = M * N;
+  hipLaunchKernelGGL(split_reduce, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, n->ws, splits, cnt, dW);
 }
 
 // Forward pass of `batch` rows already in H[0]; output pre-activations in Z[L-1].
@@ -302,10 +446,9 @@ int backward(ymc_mlp *n, int batch) {
   for (int l = L - 1; l >= 0; --l) {
     const int in = n->widths[l], ow = n->widths[l + 1];
     // bias gradient from the FP32 dZ, weight gradient dW = H[l]^T dZ
-    hipLaunchKernelGGL(column_sum, dim3((ow + 255) / 256), dim3(256), 0, n->stream, n->dZ, batch, ow,
+    hipLaunchKernelGGL(column_sum, dim3((ow + 31) / 32), dim3(256), 0, n->stream, n->dZ, batch, ow,
                        n->g + n->b_off[l]);
-    gemm(n->stream, kStoreF32, n->H[l], in, 1, n->dZh, ow, 0, in, ow, batch, nullptr, n->g + n->w_off[l], ow, nullptr,
-         0);
+    weight_grad(n, n->H[l], in, n->dZh, ow, batch, n->g + n->w_off[l]);
     if (l == 0) break;
     // dA_prev = dZ W^T, then dZ_prev = dA_prev * gelu'(Z_prev)
     gemm(n->stream, kStoreF32, n->dZh, ow, 0, n->ph + n->w_off[l], ow, 1, batch, in, ow, nullptr, n->dA, in, nullptr, 0);
@@ -314,6 +457,72 @@ int backward(ymc_mlp *n, int batch) {
                        n->dZh, c2);
   }
   return blocks;
+}
+
+bool fused_ok(const ymc_mlp *n) {
+  if ((int)n->widths.size() - 1 > kFusedLayers) return false;
+  for (int w : n->widths)
+    if (w > kFusedMax) return false;
+  return true;
+}
+
+// Rebuilds the transposed weight copy from the BF16 weights if they changed.
+ymc_status refresh_wt(ymc_mlp *n) {
+  if (!n->wt_dirty) return YMC_OK;
+  const int L = (int)n->widths.size() - 1;
+  if (!n->wt) {
+    size_t total = 0;
+    for (int l = 0; l < L; ++l) {
+      n->k16.push_back((n->widths[l] + 15) / 16 * 16);
+      n->npad.push_back((n->widths[l + 1] + 31) / 32 * 32);
+      n->wt_off.push_back(total);
+      total += (size_t)n->k16[l] * n->npad[l];
+    }
+    ymc_status st;
+    if ((st = dalloc(&n->wt, total))) return st;
+  }
+  for (int l = 0; l < L; ++l) {
+    const int cnt = n->k16[l] * n->npad[l];
+    hipLaunchKernelGGL(transpose_pad, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, n->ph + n->w_off[l],
+                       n->widths[l], n->widths[l + 1], n->wt + n->wt_off[l], n->k16[l], n->npad[l]);
+  }
+  MLP_TRY(hipGetLastError());
+  MLP_TRY(hipStreamSynchronize(n->stream));
+  n->wt_dirty = false;
+  return YMC_OK;
+}
+
+// Arguments of the fused pass for `count` networks of identical widths.
+FusedArgs fused_args(ymc_mlp *const *nets, int count, const float *x, int rows, float *const *y) {
+  FusedArgs a{};
+  const ymc_mlp *n0 = nets[0];
+  a.L = (int)n0->widths.size() - 1;
+  a.in = n0->widths[0];
+  a.rows = rows;
+  a.x = x;
+  for (int q = 0; q < count; ++q) {
+    const ymc_mlp *n = nets[q];
+    for (int l = 0; l < a.L; ++l) {
+      a.net[q].layer[l].wt = n->wt + n->wt_off[l];
+      a.net[q].layer[l].bias = n->p + n->b_off[l];
+      a.net[q].layer[l].k16 = n->k16[l];
+      a.net[q].layer[l].n = n->widths[l + 1];
+      a.net[q].layer[l].npad = n->npad[l];
+    }
+    a.net[q].y = y[q];
+  }
+  return a;
+}
+
+ymc_status check_ensemble(ymc_mlp *const *nets, int32_t count) {
+  if (!nets || count < 1 || count > kFusedNets) return fail(YMC_ERR_ARG, "ensemble of 1..%d networks expected", kFusedNets);
+  for (int q = 0; q < count; ++q) {
+    if (!nets[q]) return fail(YMC_ERR_ARG, "null network %d", q);
+    if (nets[q]->widths != nets[0]->widths || nets[q]->device != nets[0]->device)
+      return fail(YMC_ERR_ARG, "ensemble networks must share widths and device");
+  }
+  if (!fused_ok(nets[0])) return fail(YMC_ERR_ARG, "fused pass needs widths <= %d and <= %d layers", kFusedMax, kFusedLayers);
+  return YMC_OK;
 }
 
 ymc_status read_loss(ymc_mlp *n, int parts, float *loss) {
@@ -353,6 +562,12 @@ ymc_status ymc_mlp_create(int32_t device, const int32_t *widths, int32_t n_width
     ymc_mlp_destroy(n);
     return fail(YMC_ERR_NOMEM, "ymc_mlp_create: allocation failed");
   }
+  size_t wmax = 0;
+  for (int l = 0; l < L; ++l) wmax = std::max(wmax, (size_t)widths[l] * widths[l + 1]);
+  if ((st = dalloc(&n->ws, (size_t)kSplits * wmax))) {
+    ymc_mlp_destroy(n);
+    return st;
+  }
   // He-style initialisation from a seeded host generator; biases zero
   std::vector<float> init(n->n_params, 0.0f);
   std::mt19937_64 rng(seed);
@@ -377,6 +592,9 @@ void ymc_mlp_destroy(ymc_mlp *n) {
     if (q) (void)hipFree(q);
   for (float *z : n->Z) (void)hipFree(z);
   for (uint16_t *h : n->H) (void)hipFree(h);
+  if (n->wt) (void)hipFree(n->wt);
+  if (n->ws) (void)hipFree(n->ws);
+  if (n->xf) (void)hipFree(n->xf);
   if (n->stream) (void)hipStreamDestroy(n->stream);
   delete n;
 }
@@ -393,6 +611,7 @@ ymc_status ymc_mlp_set_params(ymc_mlp *n, const float *params, uint64_t count) {
   MLP_TRY(hipGetLastError());
   MLP_TRY(hipStreamSynchronize(n->stream));
   n->adam_t = 0;
+  n->wt_dirty = true;
   return YMC_OK;
 }
 
@@ -404,8 +623,47 @@ ymc_status ymc_mlp_get_params(ymc_mlp *n, float *params, uint64_t count) {
   return YMC_OK;
 }
 
+// Fused pass over `rows` host rows for `count` networks, in launches of kFusedChunk rows.
+constexpr int kFusedChunk = 4096;
+
+static ymc_status fused_forward(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y) {
+  ymc_mlp *n0 = nets[0];
+  MLP_TRY(hipSetDevice(n0->device));
+  ymc_status st;
+  for (int q = 0; q < count; ++q) {
+    if ((st = refresh_wt(nets[q]))) return st;
+    MLP_TRY(hipStreamSynchronize(nets[q]->stream));
+  }
+  const int L = (int)n0->widths.size() - 1;
+  const int in = n0->widths[0], out = n0->widths[L];
+  if (n0->xf_rows < kFusedChunk) {
+    if (n0->xf) (void)hipFree(n0->xf);
+    n0->xf = nullptr;
+    n0->xf_rows = 0;
+    if ((st = dalloc(&n0->xf, (size_t)kFusedChunk * (in + kFusedNets * out)))) return st;
+    n0->xf_rows = kFusedChunk;
+  }
+  float *yd = n0->xf + (size_t)kFusedChunk * in;
+  for (int64_t r0 = 0; r0 < rows; r0 += kFusedChunk) {
+    const int b = (int)std::min<int64_t>(kFusedChunk, rows - r0);
+    MLP_TRY(hipMemcpyAsync(n0->xf, x + r0 * in, (size_t)b * in * sizeof(float), hipMemcpyHostToDevice, n0->stream));
+    float *ys[kFusedNets];
+    for (int q = 0; q < count; ++q) ys[q] = yd + (size_t)q * kFusedChunk * out;
+    const FusedArgs a = fused_args(nets, count, n0->xf, b, ys);
+    hipLaunchKernelGGL(mlp_forward_fused, dim3((b + kFusedRows - 1) / kFusedRows, count), dim3(256), 0, n0->stream, a);
+    for (int q = 0; q < count; ++q)
+      MLP_TRY(hipMemcpyAsync(y + ((size_t)q * rows + r0) * out, ys[q], (size_t)b * out * sizeof(float),
+                             hipMemcpyDeviceToHost, n0->stream));
+  }
+  MLP_TRY(hipGetLastError());
+  MLP_TRY(hipStreamSynchronize(n0->stream));
+  return YMC_OK;
+}
+
 ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y) {
   if (!n || !x || !y || rows < 0) return fail(YMC_ERR_ARG, "ymc_mlp_forward: bad arguments");
+  if (fused_ok(n)) return fused_forward(&n, 1, x, rows, y);
+  // layer-by-layer products for networks wider than the fused pass supports
   MLP_TRY(hipSetDevice(n->device));
   const int L = (int)n->widths.size() - 1;
   const int in = n->widths[0], out = n->widths[L];
@@ -428,19 +686,33 @@ ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y) {
   return YMC_OK;
 }
 
-ymc_status ymc_mlp_time_forward(ymc_mlp *n, int32_t rows, int32_t reps, double *ms_per_pass) {
-  if (!n || !ms_per_pass || rows < 1 || reps < 1) return fail(YMC_ERR_ARG, "ymc_mlp_time_forward: bad arguments");
-  MLP_TRY(hipSetDevice(n->device));
-  ymc_status st;
-  if ((st = ensure_capacity(n, rows))) return st;
-  MLP_TRY(hipMemsetAsync(n->H[0], 0, (size_t)rows * n->widths[0] * sizeof(uint16_t), n->stream));
+ymc_status ymc_mlp_forward_ensemble(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y) {
+  ymc_status st = check_ensemble(nets, count);
+  if (st) return st;
+  if (!x || !y || rows < 0) return fail(YMC_ERR_ARG, "ymc_mlp_forward_ensemble: bad arguments");
+  return fused_forward(nets, count, x, rows, y);
+}
+
+ymc_status ymc_mlp_time_forward(ymc_mlp *const *nets, int32_t count, int32_t rows, int32_t reps, double *ms_per_pass) {
+  ymc_status st = check_ensemble(nets, count);
+  if (st) return st;
+  if (!ms_per_pass || rows < 1 || rows > kFusedChunk || reps < 1)
+    return fail(YMC_ERR_ARG, "ymc_mlp_time_forward: 1..%d rows, reps >= 1", kFusedChunk);
+  ymc_mlp *n0 = nets[0];
+  // one chunk through the host path allocates the staging buffers and refreshes the weights
+  const int L = (int)n0->widths.size() - 1;
+  std::vector<float> xh((size_t)rows * n0->widths[0], 0.0f), yh((size_t)count * rows * n0->widths[L]);
+  if ((st = fused_forward(nets, count, xh.data(), rows, yh.data()))) return st;
+  float *ys[kFusedNets];
+  for (int q = 0; q < count; ++q) ys[q] = n0->xf + (size_t)kFusedChunk * n0->widths[0] + (size_t)q * kFusedChunk * n0->widths[L];
+  const FusedArgs a = fused_args(nets, count, n0->xf, rows, ys);
+  const dim3 grid((rows + kFusedRows - 1) / kFusedRows, count);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   MLP_TRY(hipEventCreate(&e0));
   MLP_TRY(hipEventCreate(&e1));
-  forward(n, rows);  // warm-up
-  MLP_TRY(hipEventRecord(e0, n->stream));
-  for (int r = 0; r < reps; ++r) forward(n, rows);
-  MLP_TRY(hipEventRecord(e1, n->stream));
+  MLP_TRY(hipEventRecord(e0, n0->stream));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(mlp_forward_fused, grid, dim3(256), 0, n0->stream, a);
+  MLP_TRY(hipEventRecord(e1, n0->stream));
   MLP_TRY(hipGetLastError());
   MLP_TRY(hipEventSynchronize(e1));
   float ms = 0;
@@ -532,6 +804,7 @@ ymc_status ymc_mlp_train(ymc_mlp *n, int32_t epochs, int32_t batch, float lr, fl
     if (epoch_loss) epoch_loss[ep] = (float)loss_sum;
   }
   MLP_TRY(hipStreamSynchronize(n->stream));
+  n->wt_dirty = true;
   return YMC_OK;
 }
 
