@@ -223,7 +223,12 @@ __global__ void gather_rows(const float *X, const float *Y, const int *idx, int 
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-constexpr int kFusedRows = 64, kFusedMax = 256, kFusedPitch = kFusedMax + 8, kFusedLayers = 8, kFusedNets = 8;
+// kFusedRowTiles row tiles of 32 per workgroup; the 4 waves split them and the column tiles.
+constexpr int kFusedRowTiles = 1;
+constexpr int kFusedRows = 32 * kFusedRowTiles, kFusedMax = 256, kFusedPitch = kFusedMax + 8, kFusedLayers = 8,
+              kFusedNets = 8;
+constexpr int kWavesPerRowTile = 4 / kFusedRowTiles;
+constexpr int kTilesPerWave = (kFusedMax / 32 + kWavesPerRowTile - 1) / kWavesPerRowTile;
 
 struct FusedLayer {
   const uint16_t *wt;  // [npad][k16] BF16
@@ -253,34 +258,48 @@ __global__ __launch_bounds__(256) void mlp_forward_fused(FusedArgs args) {
     act[m][k] = f2bf(v);
   }
   __syncthreads();
-  // wave w takes row tile w/2 and half of the 32-column tiles
-  const int rt = wave >> 1, half = wave & 1;
+  // wave w takes row tile w / kWavesPerRowTile and its share of the 32-column tiles
+  const int rt = wave / kWavesPerRowTile, part = wave % kWavesPerRowTile;
   const int lr = lane & 31, lk = (lane >> 5) * 8;
   for (int l = 0; l < args.L; ++l) {
     const FusedLayer ly = net.layer[l];
     const int ctiles = ly.npad / 32;
-    const int cpw = (ctiles + 1) >> 1;
-    const int c0 = half * cpw;
-    const int cn = min(cpw, ctiles - c0);
-    float16_t acc[4];
-    for (int j = 0; j < 4; ++j)
+    const int cpw = (ctiles + kWavesPerRowTile - 1) / kWavesPerRowTile;
+    const int c0 = part * cpw;
+    const int cn = min(cpw, ctiles - c0);  // may be <= 0 for narrow layers: this wave then idles
+    float16_t acc[kTilesPerWave];
+    for (int j = 0; j < kTilesPerWave; ++j)
       for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
-    for (int k0 = 0; k0 < ly.k16; k0 += 16) {
-      const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&act[rt * 32 + lr][k0 + lk]);
+    // The four B fragments of a step are loaded together (tiles past cn reload the last valid
+    // tile and their products are discarded), so their latencies overlap; the step loop is
+    // unrolled for the same reason.
+    if (cn > 0) {
+      const uint16_t *wrow[kTilesPerWave];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (j < cn) {
-          const bf16x8_t b =
-              *reinterpret_cast<const bf16x8_t *>(ly.wt + (size_t)((c0 + j) * 32 + lr) * ly.k16 + k0 + lk);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+      for (int j = 0; j < kTilesPerWave; ++j)
+        wrow[j] = ly.wt + (size_t)((c0 + min(j, cn - 1)) * 32 + lr) * ly.k16 + lk;
+      bf16x8_t bn[kTilesPerWave];
+#pragma unroll
+      for (int j = 0; j < kTilesPerWave; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(wrow[j]);
+      for (int k0 = 0; k0 < ly.k16; k0 += 16) {
+        bf16x8_t b[kTilesPerWave];
+#pragma unroll
+        for (int j = 0; j < kTilesPerWave; ++j) b[j] = bn[j];
+        if (k0 + 16 < ly.k16) {  // next step's fragments in flight during this step's products
+#pragma unroll
+          for (int j = 0; j < kTilesPerWave; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(wrow[j] + k0 + 16);
         }
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&act[rt * 32 + lr][k0 + lk]);
+#pragma unroll
+        for (int j = 0; j < kTilesPerWave; ++j)
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
       }
     }
     __syncthreads();  // every wave has read this layer's input
     const bool last = l + 1 == args.L;
     const int next_k16 = last ? 0 : net.layer[l + 1].k16;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kTilesPerWave; ++j) {
       if (j >= cn) continue;
       const int col = (c0 + j) * 32 + lr;
       for (int v = 0; v < 16; ++v) {
