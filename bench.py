@@ -142,7 +142,7 @@ def main() -> int:
             "tables": tables,
             "spectra_per_s": args.steps / elapsed if tables > 1 else None,
             "launch_params": {k: params[k] for k in ("pool_lanes", "fast_min_lanes", "fast_max_iters",
-                                                      "compact_interval") if k in params},
+                                                      "chain_lanes") if k in params},
             "steps_per_photon": steps / n_photons,
             "fresnel_per_photon": fresnel / n_photons,
             "kernel_ms": kms,

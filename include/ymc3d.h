@@ -116,9 +116,13 @@ typedef struct ymc_params {
   int32_t fresnel_mode;        /* YMC_FRESNEL_YMC3D (reference formula) or _EXACT */
   int32_t kill_zero_step;      /* zero-step kill of src/mc3d_gpu.cu:343: 1 = at the end of each 45-step
                                   launch as the reference (default), 2 = every step, 0 = never */
-  int32_t chain_lanes;         /* nominal lane count of the chain schedule; 0 selects 524288 */
+  int32_t chain_lanes;         /* nominal lane count of the chain schedule; 0 selects 524288.  The
+                                  first chains hold ceil(N / (2*chain_lanes)) packets: a lane runs a
+                                  chain's packets back to back before it returns to the work queue,
+                                  so this sets how often finished lanes are refilled (the pool's
+                                  compaction interval) */
   int32_t pool_lanes;          /* resident lanes (device); 0 selects full occupancy */
-  int32_t compact_interval;    /* steps between pool compaction points; 0 selects default */
+  int32_t compact_interval;    /* loop iterations between polls of the cancel flag; 0 selects 1024 */
   uint32_t flags;              /* YMC_FLAG_* */
   int32_t fast_min_lanes;      /* fast-crossing loop runs while >= this many lanes of a wave can
                                   take a plain voxel crossing; 0 selects default, >64 disables */

@@ -1,13 +1,21 @@
 """Autotuner for the persistent transport launch.
 
-Searches the launch parameters that do not change results (tallies are bit-identical for any
-value of them, which the GPU tests check):
+Searches, on a sample budget of the actual workload:
   pool_lanes        resident lanes (workgroups x 256), up to full occupancy
   fast_min_lanes    the safe-cube crossing loop continues while this many lanes of a wave can cross
   fast_max_iters    cap on consecutive fast iterations
-  compact_interval  steps between the cancel-flag polls
-on a sample budget of the actual workload, then stores the best setting in a JSON file keyed by
-device name and workload signature.  Engine runs pick it up through `tuned(...)`.
+  chain_lanes       the compaction interval of the persistent pool.  A lane runs a chain's packets
+                    back to back, then returns to the work queue for the next chain; the guided
+                    schedule gives the first chains ceil(N / (2 chain_lanes)) packets and shrinks
+                    them to single packets towards the end.  Relative to the per-lane share of the
+                    budget, N / pool, the first chains hold pool / (2 chain_lanes) shares whatever
+                    N is, so chain_lanes is searched as a multiple of the full-occupancy pool: short
+                    chains keep lanes busy when packet costs vary between tables, long chains cost
+                    fewer queue visits and stream seedings.
+The first three do not change results (tallies are bit-identical for any value of them, which the
+GPU tests check); chain_lanes decides which stream each packet draws from, so results change at
+the statistical level and stay reproducible for the stored setting.  The best setting goes to a
+JSON file keyed by device name and workload signature; engine runs pick it up through `tuned(...)`.
 """
 from __future__ import annotations
 
@@ -22,10 +30,10 @@ import numpy as np
 CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                           "autotune.json"))
 SEARCH = {
-    "pool_frac": (1.0, 0.75, 0.5),
-    "fast_min_lanes": (4, 8, 16, 32),
+    "pool_frac": (1.0, 0.5),
+    "fast_min_lanes": (16, 32),
     "fast_max_iters": (16, 64),
-    "compact_interval": (1024,),
+    "chain_factor": (1, 4, 16),
 }
 
 
@@ -78,9 +86,9 @@ def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, ver
     engine.run()
     full = engine.pool_lanes()
     results = []
-    for pf, fm, fx, ci in itertools.product(*SEARCH.values()):
+    for pf, fm, fx, cf in itertools.product(*SEARCH.values()):
         cand = {"pool_lanes": max(256, int(full * pf) // 256 * 256), "fast_min_lanes": fm, "fast_max_iters": fx,
-                "compact_interval": ci}
+                "chain_lanes": int(full * cf)}
         engine.prepare(**params, **cand)
         ms = min(engine.run() for _ in range(repeats))
         results.append((ms, cand))
@@ -110,7 +118,8 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="tune launch parameters for a workload")
     ap.add_argument("workload", choices=["skin_vessels", "skin_slab_sphere", "batched_spectra"])
-    ap.add_argument("--sample", type=int, default=20_000_000)
+    ap.add_argument("--sample", type=int, default=0, help="packets per timed run (0: the full budget; tail "
+                                                             "effects depend on it)")
     a = ap.parse_args()
     eng = y.Engine(0)
     sc = getattr(configs, a.workload)()

@@ -16,11 +16,18 @@ TABLES_PER_LAUNCH = 2048
 ENGINES = 6
 
 
+# chain_lanes of the launches.  Packet costs differ by orders of magnitude between samples (weakly
+# absorbing samples walk far longer), so chains stay short (the first ones hold 1/32 of a lane's
+# share of the budget): a long chain of slow packets would keep its lane busy long after the rest
+# of the pool has drained.
+CHAIN_LANES = 16 * 458752
+
+
 def _launch(engine, P, photons, seed, params):
     props, geo = configs.skin2_tables(P)
     engine.set_layered(geo, configs.SKIN2_SHAPE, (configs.SKIN2_VOXEL,) * 3)
     engine.set_optprops(*props)
-    engine.prepare(**params, budgets=photons, seed=seed, chain_lanes=65536, flags=1)
+    engine.prepare(**params, budgets=photons, seed=seed, flags=1, chain_lanes=CHAIN_LANES)
     ms = engine.run()
     t = engine.read_tally()
     return ms * 1e-3, t
