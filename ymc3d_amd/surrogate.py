@@ -164,6 +164,23 @@ class Ensemble:
         for m in self.members:
             m.close()
 
+    def save(self, path: str, **extra) -> None:
+        """Parameters of every member, the target scaling and `extra` values, as an .npz file."""
+        np.savez(path, params=np.stack([m.get_params() for m in self.members]), y_mean=self.y_mean,
+                 y_std=self.y_std, **{k: np.asarray(v) for k, v in extra.items()})
+
+    @classmethod
+    def load(cls, path: str):
+        """An ensemble saved by `save`, and the extra values stored with it."""
+        with np.load(path, allow_pickle=False) as f:
+            data = {k: f[k] for k in f.files}
+        ens = cls(y_mean=data.pop("y_mean"), y_std=data.pop("y_std"))
+        for q, p in enumerate(data.pop("params")):
+            net = MLP(seed=q)
+            net.set_params(p)
+            ens.members.append(net)
+        return ens, data
+
 
 def serve(ens: Ensemble, P: np.ndarray, engine, threshold: float, fallback_photons: int, audit_frac: float = 0.0,
           seed: int = 0, **mc_kw) -> dict:

@@ -48,6 +48,9 @@ def main(argv=None) -> int:
     ap.add_argument("--fallback-photons", type=int, default=1_000_000)
     ap.add_argument("--spread-quantile", type=float, default=0.97)
     ap.add_argument("--out", default="surrogate_report.json")
+    ap.add_argument("--save-ensemble", default="", help="write the trained ensemble (.npz) after evaluation")
+    ap.add_argument("--load-ensemble", default="", help="serve a saved ensemble: skips data, training and "
+                                                        "evaluation")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
 
@@ -57,6 +60,13 @@ def main(argv=None) -> int:
 
     rep = {"config": vars(a)}
     eng = y.Engine(0)
+    if a.load_ensemble:
+        ens, extra = S.Ensemble.load(a.load_ensemble)
+        rep["loaded"] = {k: v.tolist() for k, v in extra.items()}
+        threshold = float(extra["threshold"])
+        n_star = float(extra["n_star"])
+        rep["data"] = {"packets_per_s": float(extra["mc_packets_per_s"])}
+        return serve_and_compare(a, rep, eng, ens, threshold, n_star)
 
     # 1. training data
     P, mc = D.generate(eng, a.samples, a.photons, seed=a.seed, label="train-mc")
@@ -92,6 +102,20 @@ def main(argv=None) -> int:
     threshold = float(np.quantile(spread_t.max(1), a.spread_quantile))
     rep["evaluation"]["spread_threshold_log10"] = threshold
     print("evaluation", json.dumps(rep["evaluation"]), flush=True)
+    if a.save_ensemble:
+        ens.save(a.save_ensemble, threshold=threshold, n_star=n_star, mc_packets_per_s=rep["data"]["packets_per_s"])
+    return serve_and_compare(a, rep, eng, ens, threshold, n_star)
+
+
+def serve_and_compare(a, rep, eng, ens, threshold, n_star) -> int:
+    from ymc3d_amd import surrogate as S, surrogate_data as D
+
+    if a.cube[0] <= 0 or a.cube[1] <= 0:
+        with open(a.out, "w") as f:
+            json.dump(rep, f, indent=1)
+        ens.close()
+        eng.close()
+        return 0
 
     # 4. serving a hyperspectral cube with Monte Carlo fallback
     H, W = a.cube
