@@ -253,3 +253,32 @@ def test_one_shot_run_matches_engine(engine, multipliers):
     assert got.totals.tobytes() == want.totals.tobytes()
     assert np.array_equal(got.absorption, want_abs)
     assert list(got.totals["photons"]) == [30_000, 7_000]
+
+
+def test_byte_and_word_voxel_storage_agree(engine):
+    """Grids whose rows fit 3 bits are stored one byte per voxel (row and safe radius packed),
+    others as 16-bit words.  The slab-with-inclusion grid with half its slab voxels relabelled to
+    an extra row 8 (same optical properties as row 1, so the same physics) takes the word path; its
+    tallies must equal the byte path's bit for bit."""
+    s = configs.skin_slab_sphere(photons=100_000)
+    params = dict(s.params)
+    rows = s.rows.copy()
+    mua, mus, g, n = s.optprops()
+    engine.set_geometry(rows, *s.voxel)
+    engine.set_optprops(mua, mus, g, n)
+    engine.prepare(**params, flags=y.FLAG_COUNTERS)
+    engine.run()
+    byte = engine.read_tally()
+    wide = rows.copy()
+    wide[:, :, : wide.shape[2] // 2][wide[:, :, : wide.shape[2] // 2] == 1] = 8
+    pad = lambda v: np.concatenate([v, np.repeat(v[:, 1:2], 7, axis=1)], axis=1)  # rows 3..9 copy row 1
+    engine.set_geometry(wide, *s.voxel)
+    engine.set_optprops(pad(mua), pad(mus), pad(g), pad(n))
+    engine.prepare(**params, flags=y.FLAG_COUNTERS)
+    engine.run()
+    word = engine.read_tally()
+    for f in ("R_w", "R_w2", "R_n", "T_w", "T_w2", "T_n"):
+        assert np.array_equal(getattr(byte, f), getattr(word, f)), f
+    t0, t1 = byte.totals[0], word.totals[0]
+    for f in ("photons", "R_w", "T_w", "A_w", "steps", "fresnel"):
+        assert t0[f] == t1[f], f

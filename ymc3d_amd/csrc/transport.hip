@@ -12,8 +12,9 @@
 // issued on CDNA4:
 //  - optical properties of every (table, tissue) row sit in LDS with derived constants
 //    (mu_t, mu_a/mu_t, 1/mu_t, HG terms) precomputed on the host;
-//  - the voxel grid is uint8 in global memory (read through L2), clamped like the reference's
-//    texture (src/mc3d_gpu.cu:416-418);
+//  - the voxel grid is one byte per voxel in global memory (read through L2; row in bits 0-2 and
+//    the safe radius below in bits 3-7, 16-bit words for grids with 8 or more rows), clamped like
+//    the reference's texture (src/mc3d_gpu.cu:416-418);
 //  - the boundary distance uses cached reciprocals of the direction cosines, updated only
 //    when the direction changes;
 //  - the next voxel index comes from the face test of the DDA instead of three floor divisions;
@@ -103,6 +104,7 @@ struct RowConst {
 struct KArgs {
   // geometry: voxel words, tissue row in bits 0-7, safe radius in bits 8-15
   const uint16_t *vox;
+  const uint8_t *vox8;  // packed voxel bytes (row in bits 0-2, safe radius in bits 3-7) or null
   int nx, ny, nz;
   float dx, dy, dz, Lx, Ly;
   // parametric geometry per table (layered runs only)
@@ -195,7 +197,12 @@ __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int
   const uint32_t ic = (uint32_t)min(max(i, 0), a.nx - 1);
   const uint32_t jc = (uint32_t)min(max(j, 0), a.ny - 1);
   const uint32_t kc = (uint32_t)min(max(k, 0), a.nz - 1);
-  return a.vox[(kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic];
+  const uint32_t idx = (kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic;
+  if (a.vox8) {
+    const uint32_t b = a.vox8[idx];
+    return (b & 7u) | ((b >> 3) << 8);
+  }
+  return a.vox[idx];
 }
 
 // Index after a move: an axis steps when the new coordinate reached the face ahead (the face at
@@ -755,6 +762,7 @@ struct ymc_engine {
   int nx = 0, ny = 0, nz = 0;
   float dx = 0, dy = 0, dz = 0;
   bool have_geom = false;
+  bool vox8 = false;  // voxel grid stored one byte per voxel
   // parametric geometry (ymc_engine_set_layered) instead of a voxel grid
   bool layered = false;
   ymc::DevBuf layered_buf;
@@ -918,10 +926,20 @@ ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
   if (nv >= (1ull << 31)) return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: more than 2^31 voxels");
   std::vector<uint8_t> radius(nv);
   safe_radius(g->tissue, g->nx, g->ny, g->nz, radius.data());
-  std::vector<uint16_t> words(nv);
-  for (size_t v = 0; v < nv; ++v) words[v] = (uint16_t)(g->tissue[v] | (radius[v] << 8));
-  if ((st = e->vox.ensure(nv * 2))) return st;
-  HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 2, hipMemcpyHostToDevice));
+  uint8_t max_row = 0;
+  for (size_t v = 0; v < nv; ++v) max_row = std::max(max_row, g->tissue[v]);
+  e->vox8 = max_row < 8;  // rows fit 3 bits: one byte per voxel with the safe radius (<= 31) above
+  if (e->vox8) {
+    std::vector<uint8_t> bytes(nv);
+    for (size_t v = 0; v < nv; ++v) bytes[v] = (uint8_t)(g->tissue[v] | (std::min<int>(radius[v], 31) << 3));
+    if ((st = e->vox.ensure(nv))) return st;
+    HIP_TRY(hipMemcpy(e->vox.p, bytes.data(), nv, hipMemcpyHostToDevice));
+  } else {
+    std::vector<uint16_t> words(nv);
+    for (size_t v = 0; v < nv; ++v) words[v] = (uint16_t)(g->tissue[v] | (radius[v] << 8));
+    if ((st = e->vox.ensure(nv * 2))) return st;
+    HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 2, hipMemcpyHostToDevice));
+  }
   e->nx = g->nx;
   e->ny = g->ny;
   e->nz = g->nz;
@@ -1094,6 +1112,7 @@ ymc_status ymc_engine_run(ymc_engine *e) {
 
   KArgs a{};
   a.vox = e->vox.as<uint16_t>();
+  a.vox8 = (e->vox8 && !e->layered) ? e->vox.as<uint8_t>() : nullptr;
   a.layered = e->layered_buf.as<ymc_layered>();
   a.nx = e->nx;
   a.ny = e->ny;
