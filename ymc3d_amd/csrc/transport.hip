@@ -354,10 +354,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const int ni_ = next_index(p.i, p.ux, nx, bx);
   const int nj_ = next_index(p.j, p.uy, ny, by);
   const int nk_ = next_index(p.k, p.uz, nz, bz);
-  // Inside the anchor's safe cube the voxel's row is the current row: no load.
-  const bool cube = false;
-  uint32_t nw = (uint32_t)p.t;
-  if (!cube) nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
+  const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
   const int next_t = air_override ? YMC_TISSUE_AIR : (int)(nw & 0xffu);
   const float nt = tab[next_t].r0.w;
@@ -485,10 +482,10 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.j = nj_;
     p.k = nk_;
     p.t = next_t;
-    // a voxel loaded outside the old cube becomes the anchor
+    // the voxel just loaded becomes the anchor
     if (air_override)
       clear_anchor(p);
-    else if (!cube)
+    else
       set_anchor(p, ni_, nj_, nk_, (int)(nw >> 8));
   }
   if (COUNTERS) acc.steps++;
