@@ -55,6 +55,7 @@ def main() -> int:
     ap.add_argument("--chain-lanes", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--absorption", action="store_true", help="also tally the per-voxel absorption grid")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the workload, time a plain device-to-device copy of this many bytes (the "
                          "reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes)")
@@ -94,14 +95,15 @@ def main() -> int:
     scene.load(eng)
 
     # Counter pass (deterministic: the same chains and streams as every timed step).
-    eng.prepare(**params, flags=y.FLAG_COUNTERS)
+    extra = y.FLAG_ABSORPTION if args.absorption else 0
+    eng.prepare(**params, flags=y.FLAG_COUNTERS | extra)
     eng.run()
     counted = eng.read_tally()
     steps = int(counted.totals["steps"].sum())
     fresnel = int(counted.totals["fresnel"].sum())
     tables = len(counted.totals)
 
-    eng.prepare(**params)
+    eng.prepare(**params, flags=extra)
     for _ in range(args.warmup):
         eng.run()
         eng.read_tally()
@@ -141,6 +143,7 @@ def main() -> int:
             "pool_lanes": eng.pool_lanes(),
             "tables": tables,
             "spectra_per_s": args.steps / elapsed if tables > 1 else None,
+            "absorption_grid": bool(args.absorption),
             "launch_params": {k: params[k] for k in ("pool_lanes", "fast_min_lanes", "fast_max_iters",
                                                       "chain_lanes") if k in params},
             "steps_per_photon": steps / n_photons,
@@ -153,7 +156,7 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": MEASURED_TRAFFIC.get((args.workload, n_photons, eng.pool_lanes())),
+            "traffic": None if args.absorption else MEASURED_TRAFFIC.get((args.workload, n_photons, eng.pool_lanes())),
         },
         "cpu_baseline": None,
     }
