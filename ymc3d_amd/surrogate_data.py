@@ -1,7 +1,8 @@
 """Monte Carlo data for the reflectance surrogate: batched runs of the two-layer-plus-vessel skin
-model (configs.skin2_*), one parametric geometry and optical-property table per sample, 64
-samples per persistent launch, each with its own packet budget, pencil beam at the surface centre,
-32 radial reflectance bins plus total reflectance per sample."""
+model (configs.skin2_*), one parametric geometry and optical-property table per sample, thousands
+of samples per persistent launch (each with its own packet budget) and several launches in flight
+on separate engines, pencil beam at the surface centre, 32 radial reflectance bins plus total
+reflectance per sample."""
 from __future__ import annotations
 
 import time

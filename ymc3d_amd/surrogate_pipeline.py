@@ -16,6 +16,9 @@ Steps:
     measure the served error;
  5. with vs without the surrogate: spectra/s of surrogate serving (fallback included) against plain
     Monte Carlo at matched accuracy.
+
+`--cube 0 0 --save-ensemble e.npz` stops after step 3 and stores the ensemble with its threshold and
+the Monte Carlo figures step 5 needs; `--load-ensemble e.npz` then runs steps 4-5 alone.
 """
 from __future__ import annotations
 
