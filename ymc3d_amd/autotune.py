@@ -31,7 +31,7 @@ CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.
                                                           "autotune.json"))
 SEARCH = {
     "pool_frac": (1.0, 0.5),
-    "fast_min_lanes": (16, 32),
+    "fast_min_lanes": (16, 32, 48),
     "fast_max_iters": (16, 64),
     "chain_factor": (1, 4, 16),
 }
@@ -118,9 +118,10 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="tune launch parameters for a workload")
     ap.add_argument("workload", choices=["skin_vessels", "skin_slab_sphere", "batched_spectra"])
+    ap.add_argument("--photons", type=int, default=0, help="budget of the tuned scene (0: the scene default)")
     ap.add_argument("--sample", type=int, default=0, help="packets per timed run (0: the full budget; tail "
                                                              "effects depend on it)")
     a = ap.parse_args()
     eng = y.Engine(0)
-    sc = getattr(configs, a.workload)()
+    sc = getattr(configs, a.workload)(photons=a.photons) if a.photons else getattr(configs, a.workload)()
     print("best:", json.dumps(tune(eng, sc, a.sample)))

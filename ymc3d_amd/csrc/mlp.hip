@@ -165,22 +165,25 @@ __global__ void mse_grad(const float *out, const float *y, float *dOut, uint16_t
   if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0];
 }
 
-// db[c] = sum over rows of dZ[r][c] (FP32).  A workgroup of 256 threads sums 32 columns: 8 row
-// groups per column (coalesced reads of 32 consecutive columns), then a fixed-order reduction of
-// the 8 partial sums, so the result does not depend on scheduling.
-__global__ void column_sum(const float *dZ, int rows, int cols, float *db) {
-  __shared__ float part[8][32];
+// Partial column sums of dZ over a chunk of kSumRows rows: workgroup (x, y) sums 32 columns of
+// rows [y*kSumRows, (y+1)*kSumRows) with 8 row groups (coalesced reads of 32 consecutive columns)
+// and writes part[y][c]; a fixed-order reduction over chunks (split_reduce) gives db, so the result
+// does not depend on scheduling.
+constexpr int kSumRows = 64;
+__global__ void column_sum(const float *dZ, int rows, int cols, float *part) {
+  __shared__ float acc[8][32];
   const int cl = threadIdx.x & 31, rg = threadIdx.x >> 5;
   const int c = blockIdx.x * 32 + cl;
+  const int r0 = blockIdx.y * kSumRows, r1 = min(rows, r0 + kSumRows);
   float s = 0.0f;
   if (c < cols)
-    for (int r = rg; r < rows; r += 8) s += dZ[(size_t)r * cols + c];
-  part[rg][cl] = s;
+    for (int r = r0 + rg; r < r1; r += 8) s += dZ[(size_t)r * cols + c];
+  acc[rg][cl] = s;
   __syncthreads();
   if (rg == 0 && c < cols) {
     float t = 0.0f;
-    for (int q = 0; q < 8; ++q) t += part[q][cl];
-    db[c] = t;
+    for (int q = 0; q < 8; ++q) t += acc[q][cl];
+    part[(size_t)blockIdx.y * cols + c] = t;
   }
 }
 
@@ -367,7 +370,8 @@ struct ymc_mlp {
   std::vector<size_t> wt_off;
   std::vector<int> k16, npad;
   bool wt_dirty = true;
-  float *ws = nullptr;  // split-K partial weight gradients [kSplits][max in*out]
+  float *ws = nullptr;  // partial sums: split-K weight gradients, per-chunk bias gradients
+  size_t ws_size = 0;   // floats
   float *xf = nullptr;  // fused-pass input staging [4096 x in]
   int xf_rows = 0;
 };
@@ -383,10 +387,27 @@ ymc_status dalloc(T **p, size_t count) {
   return YMC_OK;
 }
 
+// Largest number of split-K partial products of a weight gradient.
+constexpr int kSplits = 8;
+
 int max_width(const ymc_mlp *n) { return *std::max_element(n->widths.begin(), n->widths.end()); }
 
 ymc_status ensure_capacity(ymc_mlp *n, int batch) {
   if (batch <= n->cap) return YMC_OK;
+  {  // partial-sum workspace: kSplits weight-gradient slabs, or one bias-gradient row per row chunk
+    const int L = (int)n->widths.size() - 1;
+    size_t wmax = 0;
+    for (int l = 0; l < L; ++l) wmax = std::max(wmax, (size_t)n->widths[l] * n->widths[l + 1]);
+    const size_t need = std::max((size_t)kSplits * wmax, (size_t)((batch + kSumRows - 1) / kSumRows) * max_width(n));
+    if (need > n->ws_size) {
+      (void)hipFree(n->ws);
+      n->ws = nullptr;
+      n->ws_size = 0;
+      ymc_status st;
+      if ((st = dalloc(&n->ws, need))) return st;
+      n->ws_size = need;
+    }
+  }
   const int L = (int)n->widths.size() - 1;
   for (float *z : n->Z) (void)hipFree(z);
   for (uint16_t *h : n->H) (void)hipFree(h);
@@ -428,7 +449,6 @@ void gemm(hipStream_t s, int epi, const uint16_t *A, int lda, int tA, const uint
 
 // Weight gradient dW [M x N] = X^T dZ over `batch` rows: split over K (the batch) into up to
 // kSplits partial products reduced in a fixed order when the output has few 64x64 tiles.
-constexpr int kSplits = 8;
 void weight_grad(ymc_mlp *n, const uint16_t *X, int M, const uint16_t *dZ, int N, int batch, float *dW);
 
 void weight_grad(ymc_mlp *n, const uint16_t *X, int M, const uint16_t *dZ, int N, int batch, float *dW) {
@@ -465,7 +485,9 @@ int backward(ymc_mlp *n, int batch) {
   for (int l = L - 1; l >= 0; --l) {
     const int in = n->widths[l], ow = n->widths[l + 1];
     // bias gradient from the FP32 dZ, weight gradient dW = H[l]^T dZ
-    hipLaunchKernelGGL(column_sum, dim3((ow + 31) / 32), dim3(256), 0, n->stream, n->dZ, batch, ow,
+    const int chunks = (batch + kSumRows - 1) / kSumRows;
+    hipLaunchKernelGGL(column_sum, dim3((ow + 31) / 32, chunks), dim3(256), 0, n->stream, n->dZ, batch, ow, n->ws);
+    hipLaunchKernelGGL(split_reduce, dim3((ow + 255) / 256), dim3(256), 0, n->stream, n->ws, chunks, ow,
                        n->g + n->b_off[l]);
     weight_grad(n, n->H[l], in, n->dZh, ow, batch, n->g + n->w_off[l]);
     if (l == 0) break;
@@ -581,12 +603,7 @@ ymc_status ymc_mlp_create(int32_t device, const int32_t *widths, int32_t n_width
     ymc_mlp_destroy(n);
     return fail(YMC_ERR_NOMEM, "ymc_mlp_create: allocation failed");
   }
-  size_t wmax = 0;
-  for (int l = 0; l < L; ++l) wmax = std::max(wmax, (size_t)widths[l] * widths[l + 1]);
-  if ((st = dalloc(&n->ws, (size_t)kSplits * wmax))) {
-    ymc_mlp_destroy(n);
-    return st;
-  }
+
   // He-style initialisation from a seeded host generator; biases zero
   std::vector<float> init(n->n_params, 0.0f);
   std::mt19937_64 rng(seed);
