@@ -17,7 +17,8 @@
 //    the reference's texture (src/mc3d_gpu.cu:416-418);
 //  - the boundary distance uses cached reciprocals of the direction cosines, updated only
 //    when the direction changes;
-//  - the next voxel index comes from the face test of the DDA instead of three floor divisions;
+//  - the next voxel index comes from the face test of the DDA instead of three floor divisions
+//    (the divisions return only for the rare packet whose x or y index lies outside the grid);
 //  - the HG, roulette and step-length math runs only for lanes that interact, behind a
 //    wave-uniform branch; lanes that only cross a voxel face still advance the stream by the
 //    four draws the reference consumes on every step;
@@ -350,10 +351,22 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   float nz = (hit && db == tz) ? bz : fmaf(p.uz, hop, p.z);
   const float s_left = p.s - ds;                    // :207
 
-  // Next voxel (:209-213): an axis steps when the new coordinate reached the face ahead.
-  const int ni_ = next_index(p.i, p.ux, nx, bx);
-  const int nj_ = next_index(p.j, p.uy, ny, by);
-  const int nk_ = next_index(p.k, p.uz, nz, bz);
+  // Next voxel (:209-213): an axis steps when the new coordinate reached the face ahead.  This
+  // equals the reference's grid_coord of the new position while each index names a grid voxel
+  // whose closed extent holds the position.  An x or y index outside the grid (a packet launched
+  // on the far wall, then folded back inside by a mirror wall) takes grid_coord itself, which
+  // brings the index back to the position as the reference does.
+  int ni_ = next_index(p.i, p.ux, nx, bx);
+  int nj_ = next_index(p.j, p.uy, ny, by);
+  int nk_ = next_index(p.k, p.uz, nz, bz);
+  const bool outside = ((uint32_t)p.i >= (uint32_t)a.nx) | ((uint32_t)p.j >= (uint32_t)a.ny);
+  if (__builtin_amdgcn_ballot_w64(outside) != 0) {
+    if (outside) {
+      ni_ = grid_coord(nx, p.ux, a.dx);
+      nj_ = grid_coord(ny, p.uy, a.dy);
+      nk_ = grid_coord(nz, p.uz, a.dz);
+    }
+  }
   const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
   const int next_t = air_override ? YMC_TISSUE_AIR : (int)(nw & 0xffu);
