@@ -87,7 +87,7 @@ def test_autotune_store(tmp_path, monkeypatch):
     c = configs.van_de_hulst_slab(photons=10)
     assert autotune.signature(a, a.params) != autotune.signature(c, c.params)
     assert autotune.tuned(a, a.params) == {}
-    best = {"pool_lanes": 1024, "fast_min_lanes": 32, "fast_max_iters": 16, "chain_lanes": 4096}
+    best = {"pool_lanes": 1024, "jump_max_crossings": 64, "chain_lanes": 4096}
     (tmp_path / "tune.json").write_text(json.dumps({"dev": {autotune.signature(a, a.params): {"params": best}}}))
     assert autotune.tuned(b, b.params) == best
     assert autotune.tuned(c, c.params) == {}

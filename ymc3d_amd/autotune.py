@@ -2,8 +2,8 @@
 
 Searches, on a sample budget of the actual workload:
   pool_lanes        resident lanes (workgroups x 256), up to full occupancy
-  fast_min_lanes    the safe-cube crossing loop continues while this many lanes of a wave can cross
-  fast_max_iters    cap on consecutive fast iterations
+  jump_max_crossings  most plain crossings one safe-cube flight jump may take (the wave runs the
+                    stream advance of its longest jump, so a cap trades long jumps for idle lanes)
   chain_lanes       the compaction interval of the persistent pool.  A lane runs a chain's packets
                     back to back, then returns to the work queue for the next chain; the guided
                     schedule gives the first chains ceil(N / (2 chain_lanes)) packets and shrinks
@@ -12,8 +12,9 @@ Searches, on a sample budget of the actual workload:
                     N is, so chain_lanes is searched as a multiple of the full-occupancy pool: short
                     chains keep lanes busy when packet costs vary between tables, long chains cost
                     fewer queue visits and stream seedings.
-The first three do not change results (tallies are bit-identical for any value of them, which the
-GPU tests check); chain_lanes decides which stream each packet draws from, so results change at
+pool_lanes does not change results (tallies are bit-identical for any value, which the GPU tests
+check); jump_max_crossings changes only which crossings are jumped, i.e. float rounding of the
+positions, with the same draws; chain_lanes decides which stream each packet draws from, so results change at
 the statistical level and stay reproducible for the stored setting.  The best setting goes to a
 JSON file keyed by device name and workload signature; engine runs pick it up through `tuned(...)`.
 """
@@ -31,8 +32,7 @@ CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.
                                                           "autotune.json"))
 SEARCH = {
     "pool_frac": (1.0, 0.5),
-    "fast_min_lanes": (16, 32, 48),
-    "fast_max_iters": (16, 64),
+    "jump_max_crossings": (16, 64),
     "chain_factor": (1, 4, 16),
 }
 
@@ -86,8 +86,8 @@ def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, ver
     engine.run()
     full = engine.pool_lanes()
     results = []
-    for pf, fm, fx, cf in itertools.product(*SEARCH.values()):
-        cand = {"pool_lanes": max(256, int(full * pf) // 256 * 256), "fast_min_lanes": fm, "fast_max_iters": fx,
+    for pf, jc, cf in itertools.product(*SEARCH.values()):
+        cand = {"pool_lanes": max(256, int(full * pf) // 256 * 256), "jump_max_crossings": jc,
                 "chain_lanes": int(full * cf)}
         engine.prepare(**params, **cand)
         ms = min(engine.run() for _ in range(repeats))

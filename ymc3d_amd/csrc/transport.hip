@@ -22,11 +22,11 @@
 //  - the HG, roulette and step-length math runs only for lanes that interact, behind a
 //    wave-uniform branch; lanes that only cross a voxel face still advance the stream by the
 //    four draws the reference consumes on every step;
-//  - plain voxel crossings inside a region of one tissue run in a tight inner loop with no
-//    memory access: each voxel word carries a safe radius (Chebyshev distance within which all
-//    voxels share its tissue and no mirror wall or top surface is reached), and a packet that
-//    stays inside the safe cube of its anchor voxel crosses faces without loading voxels or
-//    testing for Fresnel.  Such a crossing is exactly the general step's crossing.
+//  - plain voxel crossings inside a region of one tissue are taken together: each voxel word
+//    carries a safe radius (Chebyshev distance within which all voxels share its tissue and no
+//    mirror wall or top surface is reached), and the part of a flight that stays inside the safe
+//    cube of its anchor voxel is one jump to its last face, with no voxel load and no Fresnel
+//    test, and the stream advanced by the draws of the crossings it replaces.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,6 +108,7 @@ struct KArgs {
   const uint8_t *vox8;  // packed voxel bytes (row in bits 0-2, safe radius in bits 3-7) or null
   int nx, ny, nz;
   float dx, dy, dz, Lx, Ly;
+  float idx, idy, idz;  // 1/dx, 1/dy, 1/dz
   // parametric geometry per table (layered runs only)
   const ymc_layered *layered;
   // optical properties
@@ -141,8 +142,8 @@ struct KArgs {
   unsigned long long n_chains;
   const volatile int *cancel;
   int poll_interval;
-  int fast_min;  // inner fast-crossing loop continues while >= fast_min lanes can cross
-  int fast_max;  // and at most fast_max iterations
+  int jump_on;   // safe-cube flight jumps enabled
+  int jump_cap;  // most crossings one jump may take
   ymc_photon_record *records;
   unsigned long long n_records;
   // absorption grid [tables][nz][ny][nx] (YMC_FLAG_ABSORPTION) or null
@@ -277,45 +278,91 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
   set_anchor(p, p.i, p.j, 0, (int)(w >> 8));
 }
 
-// A plain crossing of a voxel face inside the anchor's safe cube: hit, positive step length,
-// destination inside the cube (same tissue, so ni == nt and no Fresnel draw), new position
-// strictly between the mirror walls, live packet in tissue.  For such a step the general step
-// below reduces to: move to the face, advance the indices, s -= db*mu_t, four draws.
-// Returns true and commits when the packet's next step is such a crossing.
+// Voxel of coordinate c along one axis, clamped to [lo, hi].  Rounding can put it one voxel off
+// next to a face; the jump below checks the position it commits against the voxel it names.
+__device__ __forceinline__ int axis_voxel(float c, float inv_d, int lo, int hi) {
+  const float q = floorf(c * inv_d);
+  return (int)fminf(fmaxf(q, (float)lo), (float)hi);
+}
+
+// Coordinate c with index e is a state the face test continues exactly: inside the voxel's
+// closed extent and strictly before the face ahead (so the next step has a positive length).
+__device__ __forceinline__ bool axis_ok(float c, int e, float d, bool up) {
+  const float lo = (float)e * d, hi = (float)(e + 1) * d;
+  return up ? ((c >= lo) & (c < hi)) : ((c > lo) & (c <= hi));
+}
+
+// Flight jump.  Inside the safe cube of the anchor voxel every voxel has the anchor's tissue, no
+// mirror wall and no top surface, so each face the straight path crosses there is a plain
+// crossing of the general step: hit, no Fresnel draw, new indices, s -= db*mu_t, four draws.
+// The path stops at the interaction point (optical depth s) or where it leaves the cube,
+// whichever comes first.  The jump moves the packet to the last face it crosses before that
+// point, in one step, and advances the stream by the four draws of each crossing it skips; the
+// general step then takes the interaction or the crossing out of the cube.  Positions are the
+// general step's up to rounding (one fma from the start instead of one per crossing), and the
+// draws are the same, so packets stay in lockstep with step-by-step transport.
+// Returns true and commits when the jump was taken.  Not taken: no face to cross, more than
+// jump_cap crossings, a start on a face ahead (the general step takes that zero-length step, which
+// crosses every face the packet sits on together and is subject to the zero-step kill), or a
+// rounding case the commit checks reject.  Inside the flight the step-by-step walk can also meet a
+// zero-length step, when a crossing's end position rounds onto the next face; the jump takes it
+// like the other crossings, without the zero-step kill.
 template <bool COUNTERS>
-__device__ __forceinline__ bool fast_crossing(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float mu_t,
-                                              LaneAcc &acc) {
-  const float bx = (float)(p.ux >= 0.0f ? p.i + 1 : p.i) * a.dx;
-  const float by = (float)(p.uy >= 0.0f ? p.j + 1 : p.j) * a.dy;
-  const float bz = (float)(p.uz >= 0.0f ? p.k + 1 : p.k) * a.dz;
-  const float tx = (bx - p.x) * p.iux;
-  const float ty = (by - p.y) * p.iuy;
-  const float tz = (bz - p.z) * p.iuz;
-  const float db = fminf(tx, fminf(ty, tz));
-  const float dbm = __fmul_rn(db, mu_t);  // rounded product, as in step_packet
-  const float nx = (db == tx) ? bx : fmaf(p.ux, db, p.x);
-  const float ny = (db == ty) ? by : fmaf(p.uy, db, p.y);
-  const float nz = (db == tz) ? bz : fmaf(p.uz, db, p.z);
-  const int ni = next_index(p.i, p.ux, nx, bx);
-  const int nj = next_index(p.j, p.uy, ny, by);
-  const int nk = next_index(p.k, p.uz, nz, bz);
+__device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float mu_t,
+                                            float inv_mu_t, LaneAcc &acc) {
+  const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
+  // cube of the anchor in index space: [lo, lo + r2] on each axis (empty when there is no cube)
+  const int r2 = (int)p.r2;
+  const int loi = -p.oi, loj = -p.oj, lok = -p.ok;
+  const int hii = loi + r2, hij = loj + r2, hik = lok + r2;
+  const float tex = ((float)(upx ? hii + 1 : loi) * a.dx - p.x) * p.iux;
+  const float tey = ((float)(upy ? hij + 1 : loj) * a.dy - p.y) * p.iuy;
+  const float tez = ((float)(upz ? hik + 1 : lok) * a.dz - p.z) * p.iuz;
+  const float t_stop = fminf(fminf(tex, fminf(tey, tez)), p.s * inv_mu_t);
+  // voxel at the stop point, clamped into the cube
+  const int ei = axis_voxel(fmaf(p.ux, t_stop, p.x), a.idx, loi, hii);
+  const int ej = axis_voxel(fmaf(p.uy, t_stop, p.y), a.idy, loj, hij);
+  const int ek = axis_voxel(fmaf(p.uz, t_stop, p.z), a.idz, lok, hik);
+  // last face crossed on each axis: the face entering voxel e
+  const float fx = (float)(upx ? ei : ei + 1) * a.dx;
+  const float fy = (float)(upy ? ej : ej + 1) * a.dy;
+  const float fz = (float)(upz ? ek : ek + 1) * a.dz;
+  const float tlx = (ei != p.i) ? (fx - p.x) * p.iux : 0.0f;
+  const float tly = (ej != p.j) ? (fy - p.y) * p.iuy : 0.0f;
+  const float tlz = (ek != p.k) ? (fz - p.z) * p.iuz : 0.0f;
+  const float tl = fmaxf(tlx, fmaxf(tly, tlz));
+  const float qx = (tl == tlx) ? fx : fmaf(p.ux, tl, p.x);
+  const float qy = (tl == tly) ? fy : fmaf(p.uy, tl, p.y);
+  const float qz = (tl == tlz) ? fz : fmaf(p.uz, tl, p.z);
+  const int di = upx ? ei - p.i : p.i - ei;
+  const int dj = upy ? ej - p.j : p.j - ej;
+  const int dk = upz ? ek - p.k : p.k - ek;
+  const int n = di + dj + dk;  // crossings, one step each
+  const float dbm = __fmul_rn(tl, mu_t);
+  const bool on_face = (p.x == (float)(upx ? p.i + 1 : p.i) * a.dx) | (p.y == (float)(upy ? p.j + 1 : p.j) * a.dy) |
+                       (p.z == (float)(upz ? p.k + 1 : p.k) * a.dz);
   // one predicate without short-circuit branches
-  const bool ok = (dbm <= p.s) & (db > 0.0f) & (mu_t > 0.0f) & in_cube(p, ni, nj, nk) & (nx > 0.0f) &
-                  (nx < a.Lx) & (ny > 0.0f) & (ny < a.Ly) & (p.nstep + 1 < kMaxSteps);
+  const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= a.jump_cap) &
+                  !on_face & in_cube(p, p.i, p.j, p.k) & axis_ok(qx, ei, a.dx, upx) &
+                  axis_ok(qy, ej, a.dy, upy) & axis_ok(qz, ek, a.dz, upz) & (qx > 0.0f) & (qx < a.Lx) &
+                  (qy > 0.0f) & (qy < a.Ly) & (dbm <= p.s) & (p.nstep + (uint32_t)n < kMaxSteps);
   if (ok) {
-    p.x = nx;
-    p.y = ny;
-    p.z = nz;
-    p.i = ni;
-    p.j = nj;
-    p.k = nk;
+    p.x = qx;
+    p.y = qy;
+    p.z = qz;
+    p.i = ei;
+    p.j = ej;
+    p.k = ek;
     p.s -= dbm;
+    p.nstep += (uint32_t)n;
+    if (COUNTERS) acc.steps += (uint32_t)n;
+  }
+  const int skip = ok ? n : 0;
+  for (int q = 0; q < skip; ++q) {
     mwc_next(rx, ra);
     mwc_next(rx, ra);
     mwc_next(rx, ra);
     mwc_next(rx, ra);
-    p.nstep++;
-    if (COUNTERS) acc.steps++;
   }
   return ok;
 }
@@ -654,14 +701,10 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       }
     }
     if (__ballot(alive) == 0) break;
-    // Plain crossings inside safe cubes, while enough lanes of the wave have one.
-    if (a.fast_min <= 64) {
-      const float mu_t = alive ? rows[(size_t)table * a.num_types + p.t].r1.x : 0.0f;
-      for (int it = 0; it < a.fast_max; ++it) {
-        bool f = false;
-        if (alive) f = fast_crossing<COUNTERS>(p, rx, ra, a, mu_t, acc);
-        if (__popcll(__ballot(f)) < (unsigned)a.fast_min) break;
-      }
+    // The plain crossings of the flight ahead, inside the anchor's safe cube, in one jump.
+    if (a.jump_on && alive) {
+      const float4 r1 = rows[(size_t)table * a.num_types + p.t].r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
+      jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, acc);
     }
     if (alive) {
       const RowConst *tab = rows + (size_t)table * a.num_types;
@@ -1017,6 +1060,8 @@ static ymc_status prepare_common(ymc_engine *e, const ymc_params *prm) {
         return fail(YMC_ERR_FORMAT, "layered geometry names a row outside the %d-row tables", e->num_types);
     }
   }
+  if (prm->jump_mode < 0 || prm->jump_mode > 2) return fail(YMC_ERR_ARG, "jump_mode must be 0, 1 or 2");
+  if (prm->jump_max_crossings < 0) return fail(YMC_ERR_ARG, "jump_max_crossings must be >= 0");
   e->params = *prm;
   e->tnx = prm->tally_nx > 0 ? prm->tally_nx : e->nx;
   e->tny = prm->tally_ny > 0 ? prm->tally_ny : e->ny;
@@ -1131,6 +1176,9 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   a.dy = e->dy;
   a.dz = e->dz;
   a.Lx = e->dx * (float)e->nx;
+  a.idx = 1.0f / e->dx;
+  a.idy = 1.0f / e->dy;
+  a.idz = 1.0f / e->dz;
   a.Ly = e->dy * (float)e->ny;
   a.rows = e->rows.as<RowConst>();
   a.w0 = e->w0.as<float>();
@@ -1168,8 +1216,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   *e->cancel_host = (prm.cancel && *prm.cancel) ? 1 : 0;
   a.cancel = e->cancel_dev;
   a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
-  a.fast_min = prm.fast_min_lanes > 0 ? prm.fast_min_lanes : 8;
-  a.fast_max = prm.fast_max_iters > 0 ? prm.fast_max_iters : 64;
+  a.jump_on = prm.jump_mode != 2;
+  a.jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : 64;
   a.records = e->records.as<ymc_photon_record>();
   a.n_records = e->n_records;
   a.abs_grid = e->absorbing ? e->absorb.as<unsigned long long>() : nullptr;
