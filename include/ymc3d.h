@@ -127,7 +127,9 @@ typedef struct ymc_params {
   int32_t jump_mode;           /* safe-cube flight jumps: 0 selects default (on), 1 on, 2 off.  A
                                   flight whose faces all separate voxels of one tissue moves to
                                   its last face at once, drawing what the plain crossings draw */
-  int32_t jump_max_crossings;  /* most crossings one jump may take; 0 selects 64 */
+  int32_t jump_max_crossings;  /* most crossings one jump may take; 0 selects 64.  At most 63 when
+                                  the jumped draws are skipped with the per-lane LDS table, which
+                                  the engine uses whenever it costs no occupancy */
 } ymc_params;
 
 /* Per-table totals (fixed point, scale YMC_FX_ONE). */
@@ -268,6 +270,13 @@ ymc_status ymc_engine_pool_lanes(ymc_engine *e, int32_t *lanes);
 ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t *a,
                                uint32_t n_streams, uint32_t n_draws, int32_t kind, float *out,
                                uint64_t *x_final);
+
+/* Stream-skip test hook: advances n_streams MWC streams (gpumcml_rng.cu:38-46) past the draws of
+ * `crossings` plain voxel crossings (4 draws each, crossings <= 63) with the modular skip table a
+ * flight jump uses, and returns the final states; they equal those of 4*crossings draws taken one
+ * by one.  No counterpart in the reference, whose crossings draw one at a time. */
+ymc_status ymc_engine_rng_skip(ymc_engine *e, const uint64_t *x, const uint32_t *a,
+                               uint32_t n_streams, uint32_t crossings, uint64_t *x_final);
 
 /* One-shot drop-in for run_3dmc_gpu (src/mc3d_gpu.cu:388): same arguments as plain pointers.
  * tissue holds remapped int32 rows (as geometry_t.tissue_type); optprops arrays have

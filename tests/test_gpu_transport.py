@@ -35,6 +35,19 @@ def test_rng_bit_exact(engine, multipliers):
     assert got.min() > 0.0 and got.max() <= 1.0  # oc in (0, 1]
 
 
+def test_stream_skip_bit_exact(engine, multipliers):
+    """A flight jump skips the draws of n crossings with three modular products (Montgomery, one
+    base-4 digit of n each); the states must equal those of 4n draws taken one by one, for every n a
+    jump can take and for extreme states."""
+    x, a = y.seed_streams(multipliers, 4711, 1024)
+    p = (a.astype(np.uint64) << np.uint64(32)) - np.uint64(1)
+    x[:4] = [1, (int(a[1]) - 2) << 32 | 0xFFFFFFFE, int(p[2]) - 1, (int(a[3]) - 1) << 32 | 0xFFFFFFFD]
+    for n in range(64):
+        got = engine.rng_skip(x, a, n)
+        _, want = O.mwc_fill(x, a, 4 * n, 0)
+        assert np.array_equal(got, want), n
+
+
 def _chains_one_per_packet(multipliers, n, seed):
     chains = np.zeros(n, dtype=y._abi.CHAIN_DTYPE)
     chains["count"] = 1

@@ -65,6 +65,73 @@ __device__ __forceinline__ float u32_co(uint32_t u) {
 }
 __device__ __forceinline__ float u32_oc(uint32_t u) { return 1.0f - u32_co(u); }
 
+// Stream skip.  An MWC step maps the state v = carry*2^32 + x (0 < v < p, p = a*2^32 - 1) to
+// a*x + carry, which is v*2^-32 mod p (a*2^32 = 1 mod p).  k steps multiply v by 2^-32k mod p, so
+// the draws of n skipped plain crossings (4 each) take one modular product per base-4 digit of n;
+// the lowest digit (at most 3 crossings, 12 steps) is cheaper stepped.
+// The product is Montgomery's with R = 2^64: p = -1 mod 2^32, so reducing by one 32-bit limb,
+// z -> (z + (z mod 2^32)*p) / 2^32 = a*(z mod 2^32) + z/2^32, is an MWC step itself.
+// mwc_mulmod returns v*t*2^-64 mod p for 0 <= v, t < p.
+__device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t a) {
+  const uint64_t p = ((uint64_t)a << 32) - 1;
+  const uint32_t v0 = (uint32_t)v, v1 = (uint32_t)(v >> 32), t0 = (uint32_t)t, t1 = (uint32_t)(t >> 32);
+  // z = v*t in four 32-bit limbs: z0, z1, z23
+  const uint64_t A = (uint64_t)v0 * t0;
+  const uint64_t B = (uint64_t)v0 * t1 + (A >> 32);
+  const uint64_t C = (uint64_t)v1 * t0 + (uint32_t)B;
+  const uint64_t D = (uint64_t)v1 * t1 + (B >> 32);
+  const uint32_t z0 = (uint32_t)A, z1 = (uint32_t)C;
+  const uint64_t z23 = D + (C >> 32);
+  // first limb reduction: w = a*z0 + z/2^32 (96 bits: w_hi:w_lo)
+  const uint64_t s1 = (z23 << 32) | z1;
+  const uint64_t w_lo = (uint64_t)a * z0 + s1;
+  const uint32_t w_hi = (uint32_t)(z23 >> 32) + (w_lo < s1 ? 1u : 0u);
+  // second: u = a*(w mod 2^32) + w/2^32 (65 bits: c:u), u < 3p
+  const uint64_t s2 = ((uint64_t)w_hi << 32) | (w_lo >> 32);
+  uint64_t u = (uint64_t)a * (uint32_t)w_lo + s2;
+  bool c = u < s2;
+  for (int r = 0; r < 2; ++r) {
+    if (c || u >= p) {
+      c = c && (u >= p);
+      u -= p;
+    }
+  }
+  return u;
+}
+
+// Per-lane skip table: entry 3(L-1) + d - 1 (L = 1, 2; d = 1..3) holds 2^-32k * 2^64 mod p for
+// k = 4*d*4^L draws, i.e. the state 4*d*4^L - 2 MWC steps after v = 1.  tab points at the lane's
+// first entry; entries are STRIDE apart (one column per lane of the workgroup).
+constexpr int kSkipEntries = 6;
+constexpr int kSkipMax = 63;  // crossings one skip covers (three base-4 digits)
+template <int STRIDE>
+__device__ __forceinline__ void build_skip(uint64_t *tab, uint32_t a) {
+  uint64_t st = 1;
+  int k = 0;
+#pragma unroll
+  for (int L = 1; L < 3; ++L) {
+#pragma unroll
+    for (int d = 1; d <= 3; ++d) {
+      const int want = 4 * d * (1 << (2 * L)) - 2;
+      for (; k < want; ++k) mwc_next(st, a);
+      tab[(3 * (L - 1) + d - 1) * STRIDE] = st;
+    }
+  }
+}
+// Advances the stream by the draws of n plain crossings, 0 <= n <= kSkipMax.
+template <int STRIDE>
+__device__ __forceinline__ void skip_crossings(uint64_t &rx, uint32_t ra, const uint64_t *tab, int n) {
+  const int d0 = n & 3, d1 = (n >> 2) & 3, d2 = (n >> 4) & 3;
+  for (int q = 0; q < d0; ++q) {
+    mwc_next(rx, ra);
+    mwc_next(rx, ra);
+    mwc_next(rx, ra);
+    mwc_next(rx, ra);
+  }
+  if (d1) rx = mwc_mulmod(rx, tab[(d1 - 1) * STRIDE], ra);
+  if (d2) rx = mwc_mulmod(rx, tab[(d2 + 2) * STRIDE], ra);
+}
+
 __device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 // -ln(v) for v in [2^-32, 1]: v_log_f32 (log2, no denormal range in this domain) times ln 2.
 __device__ __forceinline__ float neg_ln(float v) { return __builtin_amdgcn_logf(v) * -0.693147180559945309f; }
@@ -93,6 +160,8 @@ __device__ __forceinline__ int grid_coord(float x, float ux, float dx) {
   i = ((x > bmin) || ((x == bmin) && (ux >= 0))) ? i : i - 1;
   return i;
 }
+
+constexpr int kBlock = 256;  // threads per workgroup of the transport kernel
 
 // Per-row constants in LDS: three float4 per (table, tissue) row.
 //   r0 = {mua, mus, g, n}
@@ -144,6 +213,8 @@ struct KArgs {
   int poll_interval;
   int jump_on;   // safe-cube flight jumps enabled
   int jump_cap;  // most crossings one jump may take
+  int skip_off;  // byte offset of the per-lane stream-skip table in dynamic LDS, or -1 (then the
+                 // skipped draws are taken one by one)
   ymc_photon_record *records;
   unsigned long long n_records;
   // absorption grid [tables][nz][ny][nx] (YMC_FLAG_ABSORPTION) or null
@@ -298,7 +369,8 @@ __device__ __forceinline__ bool axis_ok(float c, int e, float d, bool up) {
 // The path stops at the interaction point (optical depth s) or where it leaves the cube,
 // whichever comes first.  The jump moves the packet to the last face it crosses before that
 // point, in one step, and advances the stream by the four draws of each crossing it skips; the
-// general step then takes the interaction or the crossing out of the cube.  Positions are the
+// general step then takes the interaction or the crossing out of the cube.  The draws are skipped
+// with the lane's skip table (up to two modular products) when the kernel has one in LDS.  Positions are the
 // general step's up to rounding (one fma from the start instead of one per crossing), and the
 // draws are the same, so packets stay in lockstep with step-by-step transport.
 // Returns true and commits when the jump was taken.  Not taken: no face to cross, more than
@@ -309,7 +381,7 @@ __device__ __forceinline__ bool axis_ok(float c, int e, float d, bool up) {
 // like the other crossings, without the zero-step kill.
 template <bool COUNTERS>
 __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float mu_t,
-                                            float inv_mu_t, LaneAcc &acc) {
+                                            float inv_mu_t, const uint64_t *skip, LaneAcc &acc) {
   const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
   // cube of the anchor in index space: [lo, lo + r2] on each axis (empty when there is no cube)
   const int r2 = (int)p.r2;
@@ -342,7 +414,8 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   const bool on_face = (p.x == (float)(upx ? p.i + 1 : p.i) * a.dx) | (p.y == (float)(upy ? p.j + 1 : p.j) * a.dy) |
                        (p.z == (float)(upz ? p.k + 1 : p.k) * a.dz);
   // one predicate without short-circuit branches
-  const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= a.jump_cap) &
+  const int cap = skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
+  const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= cap) &
                   !on_face & in_cube(p, p.i, p.j, p.k) & axis_ok(qx, ei, a.dx, upx) &
                   axis_ok(qy, ej, a.dy, upy) & axis_ok(qz, ek, a.dz, upz) & (qx > 0.0f) & (qx < a.Lx) &
                   (qy > 0.0f) & (qy < a.Ly) & (dbm <= p.s) & (p.nstep + (uint32_t)n < kMaxSteps);
@@ -357,12 +430,16 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
     p.nstep += (uint32_t)n;
     if (COUNTERS) acc.steps += (uint32_t)n;
   }
-  const int skip = ok ? n : 0;
-  for (int q = 0; q < skip; ++q) {
-    mwc_next(rx, ra);
-    mwc_next(rx, ra);
-    mwc_next(rx, ra);
-    mwc_next(rx, ra);
+  if (skip) {
+    if (ok) skip_crossings<kBlock>(rx, ra, skip, n);
+  } else {
+    const int steps = ok ? n : 0;
+    for (int q = 0; q < steps; ++q) {
+      mwc_next(rx, ra);
+      mwc_next(rx, ra);
+      mwc_next(rx, ra);
+      mwc_next(rx, ra);
+    }
   }
   return ok;
 }
@@ -609,7 +686,6 @@ __device__ __forceinline__ void tally_packet(const Packet &p, const KArgs &a, ui
   }
 }
 
-constexpr int kBlock = 256;
 
 // Byte offset of the per-table totals in the kernel's dynamic LDS (8-byte aligned).
 __host__ __device__ inline size_t lds_tot_offset(int tables, int types) {
@@ -641,6 +717,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
     __syncthreads();
   }
   const RowConst *rows = LAYERED ? a.rows : s_rows;
+  uint64_t *s_skip = a.skip_off >= 0 ? reinterpret_cast<uint64_t *>(lds_raw + a.skip_off) + threadIdx.x : nullptr;
   unsigned long long *totals = LAYERED ? a.tot : s_tot;
 
   const uint32_t lane = __lane_id();
@@ -690,6 +767,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
           table = ch.table;
           rx = a.chain_x[c];
           ra = a.chain_a[c];
+          if (s_skip) build_skip<kBlock>(s_skip, ra);
           done_in_chain = 0;
         } else {
           drained = true;
@@ -704,7 +782,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
     // The plain crossings of the flight ahead, inside the anchor's safe cube, in one jump.
     if (a.jump_on && alive) {
       const float4 r1 = rows[(size_t)table * a.num_types + p.t].r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
-      jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, acc);
+      jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, s_skip, acc);
     }
     if (alive) {
       const RowConst *tab = rows + (size_t)table * a.num_types;
@@ -750,6 +828,20 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       if (v) atomicAdd(&a.tot[q], v);
     }
   }
+}
+
+// Stream-skip test hook: each lane builds its skip table in LDS, as the transport kernel does at
+// a chain start, and skips the draws of `crossings` plain crossings.
+__global__ __launch_bounds__(kBlock) void rng_skip_kernel(const uint64_t *x, const uint32_t *a, uint32_t n,
+                                                          int crossings, uint64_t *xf) {
+  __shared__ uint64_t tab[kSkipEntries * kBlock];
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint64_t st = x[s];
+  const uint32_t m = a[s];
+  build_skip<kBlock>(tab + threadIdx.x, m);
+  skip_crossings<kBlock>(st, m, tab + threadIdx.x, crossings);
+  xf[s] = st;
 }
 
 // RNG test hook: n_streams lanes, n_draws draws each.
@@ -816,6 +908,7 @@ struct ymc_engine {
   float dx = 0, dy = 0, dz = 0;
   bool have_geom = false;
   bool vox8 = false;  // voxel grid stored one byte per voxel
+  bool last_skip_table = false;  // the last run skipped jumped draws with the LDS table
   // parametric geometry (ymc_engine_set_layered) instead of a voxel grid
   bool layered = false;
   ymc::DevBuf layered_buf;
@@ -1227,10 +1320,18 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   const bool counters = (prm.flags & YMC_FLAG_COUNTERS) != 0;
   const bool record = (prm.flags & YMC_FLAG_RECORD) != 0;
   const void *kern = select_kernel(uniform, counters, e->nr > 0, record, e->layered);
-  const size_t lds = lds_bytes(e->num_tables, e->num_types, e->layered);
-  if (lds > 64 * 1024) return fail(YMC_ERR_ARG, "optical-property tables need %zu B of LDS (max 65536)", lds);
-  int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds));
+  const size_t lds0 = lds_bytes(e->num_tables, e->num_types, e->layered);
+  if (lds0 > 64 * 1024) return fail(YMC_ERR_ARG, "optical-property tables need %zu B of LDS (max 65536)", lds0);
+  // The per-lane stream-skip table goes into LDS when it costs no occupancy.
+  const size_t skip_off = (lds0 + 7) & ~(size_t)7;
+  const size_t lds1 = skip_off + (size_t)kSkipEntries * kBlock * 8;
+  int per_cu = 0, per_cu_skip = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds0));
+  if (lds1 <= 64 * 1024) HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_skip, kern, kBlock, lds1));
+  const bool skip_table = a.jump_on && per_cu_skip >= per_cu && per_cu_skip > 0;
+  a.skip_off = skip_table ? (int)skip_off : -1;
+  const size_t lds = skip_table ? lds1 : lds0;
+  e->last_skip_table = skip_table;
   if (per_cu < 1) per_cu = 1;
   int blocks = per_cu * e->num_cus;
   if (prm.pool_lanes > 0) blocks = std::max(1, prm.pool_lanes / kBlock);
@@ -1331,6 +1432,27 @@ ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t 
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(out, dout.p, (size_t)n_streams * n_draws * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(x_final, dxf.p, (size_t)n_streams * 8, hipMemcpyDeviceToHost));
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_rng_skip(ymc_engine *e, const uint64_t *x, const uint32_t *a, uint32_t n_streams,
+                               uint32_t crossings, uint64_t *x_final) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!x || !a || !x_final || n_streams == 0) return fail(YMC_ERR_ARG, "ymc_engine_rng_skip: bad arguments");
+  if (crossings > (uint32_t)kSkipMax) return fail(YMC_ERR_ARG, "ymc_engine_rng_skip: crossings must be <= %d", kSkipMax);
+  DevBuf dx, da, dxf;
+  if ((st = dx.ensure((size_t)n_streams * 8)) || (st = da.ensure((size_t)n_streams * 4)) ||
+      (st = dxf.ensure((size_t)n_streams * 8)))
+    return st;
+  HIP_TRY(hipMemcpy(dx.p, x, (size_t)n_streams * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(da.p, a, (size_t)n_streams * 4, hipMemcpyHostToDevice));
+  const uint32_t grid = (n_streams + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(rng_skip_kernel, dim3(grid), dim3(kBlock), 0, e->stream, dx.as<uint64_t>(), da.as<uint32_t>(),
+                     n_streams, (int)crossings, dxf.as<uint64_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(x_final, dxf.p, (size_t)n_streams * 8, hipMemcpyDeviceToHost));
   return YMC_OK;
 }

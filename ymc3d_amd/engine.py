@@ -327,6 +327,16 @@ class Engine:
                                             ptr(out, C.c_float), ptr(xf, C.c_uint64)), "ymc_engine_rng_fill")
         return out, xf
 
+    def rng_skip(self, x: np.ndarray, a: np.ndarray, crossings: int) -> np.ndarray:
+        """Device stream skip past the draws of `crossings` plain crossings (4 each, <= 63), with the
+        modular table a flight jump uses; returns the final states."""
+        xx = np.ascontiguousarray(x, dtype=np.uint64)
+        aa = np.ascontiguousarray(a, dtype=np.uint32)
+        xf = np.empty_like(xx)
+        check(self._lib.ymc_engine_rng_skip(self._h, ptr(xx, C.c_uint64), ptr(aa, C.c_uint32), xx.size, crossings,
+                                            ptr(xf, C.c_uint64)), "ymc_engine_rng_skip")
+        return xf
+
 
 def layered_voxelize(geom, shape: tuple[int, int, int]) -> np.ndarray:
     """Rows [nz, ny, nx] of the voxel grid one parametric geometry describes."""
