@@ -218,9 +218,18 @@ ymc_status ymc_layered_voxelize(const ymc_layered *g, int32_t nx, int32_t ny, in
 
 /* Safe radius per voxel (host): the largest r <= 31 such that every voxel within Chebyshev
  * distance r has the same row, lies inside the lateral grid and not above the top layer (below
- * the grid the bottom layer repeats).  The engine packs it into its voxel words; exported as a
- * test hook.  No counterpart in the reference. */
+ * the grid the bottom layer repeats).  Exported as a test hook.  No counterpart in the
+ * reference. */
 ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *out);
+
+/* Safe box per voxel (host): half-extents rz <= 31 (vertical) and rxy <= 255 (lateral, square)
+ * such that every voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-rz, k+rz] has the same row, lies
+ * inside the lateral grid and not above the top layer (below the grid the bottom layer repeats).
+ * Of two candidates, (safe radius, widest square at that height) and (full vertical run, widest
+ * square at that height), those containing the safe-radius cube, the one with the larger harmonic
+ * mean of rz+1, rxy+1, rxy+1 is kept.  The engine packs it into its voxel words and a
+ * flight jump stays inside it; exported as a test hook.  No counterpart in the reference. */
+ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *rz, uint8_t *rxy);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Device engine.                                                                             */

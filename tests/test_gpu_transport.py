@@ -310,12 +310,12 @@ def test_one_shot_run_matches_engine(engine, multipliers):
     assert list(got.totals["photons"]) == [30_000, 7_000]
 
 
-def test_byte_and_word_voxel_storage_agree(engine):
-    """Grids whose rows fit 3 bits are stored one byte per voxel (row and safe radius packed),
-    others as 16-bit words.  The slab-with-inclusion grid with half its slab voxels relabelled to
-    an extra row 8 (same optical properties as row 1, so the same physics) takes the word path.
-    With flight jumps off its tallies must equal the byte path's bit for bit.  With jumps on the
-    relabelling splits safe cubes (other crossings are jumped, positions differ by rounding, draws
+def test_16_and_32_bit_voxel_words_agree(engine):
+    """Grids whose rows fit 3 bits are stored as 16-bit voxel words (row and safe box packed),
+    others as 32-bit words.  The slab-with-inclusion grid with half its slab voxels relabelled to
+    an extra row 8 (same optical properties as row 1, so the same physics) takes the 32-bit path.
+    With flight jumps off its tallies must equal the 16-bit path's bit for bit.  With jumps on the
+    relabelling splits safe boxes (other crossings are jumped, positions differ by rounding, draws
     are the same), so the totals agree far inside one standard error."""
     s = configs.skin_slab_sphere(photons=100_000)
     params = dict(s.params)

@@ -60,3 +60,71 @@ def test_safe_radius_uniform_block_and_cap():
     assert r[35, 35, 35] == 31
     assert r[5, 35, 35] == 5  # distance to the top layer
     assert r[39, 35, 35] == 31  # bottom repeats below the grid
+
+
+def lib_box(rows):
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    nz, ny, nx = rows.shape
+    rz, rxy = np.zeros_like(rows), np.zeros_like(rows)
+    assert y.load_library().ymc_safe_box(ptr(rows, C.c_uint8), nx, ny, nz, ptr(rz, C.c_uint8), ptr(rxy, C.c_uint8)) == 0
+    return rz, rxy
+
+
+def _box_ok(rows, k, j, i, z, s):
+    """Box [i-s, i+s] x [j-s, j+s] x [k-z, k+z]: voxel's row everywhere, inside the lateral grid,
+    not above the top layer, bottom layer repeating below the grid."""
+    nz, ny, nx = rows.shape
+    if i - s < 0 or i + s > nx - 1 or j - s < 0 or j + s > ny - 1 or k - z < 0:
+        return False
+    ks = np.minimum(np.arange(k - z, k + z + 1), nz - 1)
+    return bool(np.all(rows[ks][:, j - s:j + s + 1, i - s:i + s + 1] == rows[k, j, i]))
+
+
+def test_safe_box_matches_definition():
+    """Brute force on the definition: each voxel's box holds its row only and stays in bounds; it
+    contains the safe-radius cube; its lateral extent is the widest square at its height; and it is
+    the better, by the harmonic mean of rz+1, rxy+1, rxy+1, of the two candidate heights (the safe
+    radius and the full vertical run of the row) whose box contains the cube."""
+    rng = np.random.default_rng(1)
+    for shape in ((9, 11, 13), (16, 12, 10), (12, 9, 21)):
+        rows = np.ones(shape, dtype=np.uint8)
+        nz = shape[0]
+        rows[: rng.integers(1, 4)] = 3  # a top layer
+        for _ in range(2):
+            lo = [rng.integers(0, s) for s in shape]
+            hi = [min(s, l + rng.integers(1, 4)) for s, l in zip(shape, lo)]
+            rows[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = rng.integers(0, 4)
+        rz, rxy = lib_box(rows)
+        cube = brute(rows)
+        for k in range(shape[0]):
+            for j in range(shape[1]):
+                for i in range(shape[2]):
+                    z, s, r = int(rz[k, j, i]), int(rxy[k, j, i]), int(cube[k, j, i])
+                    assert _box_ok(rows, k, j, i, z, s), (shape, k, j, i, z, s)
+                    assert z >= r and s >= r
+                    # full vertical run of the row around the voxel (capped at 31)
+                    ez = 0
+                    while ez < 31 and k - ez - 1 >= 0 and rows[k - ez - 1, j, i] == rows[k, j, i] and \
+                            rows[min(k + ez + 1, nz - 1), j, i] == rows[k, j, i]:
+                        ez += 1
+
+                    def widest(zz):
+                        w = 0
+                        while w < 255 and _box_ok(rows, k, j, i, zz, w + 1):
+                            w += 1
+                        return w
+
+                    score = lambda zz, ww: 1.0 / (2.0 / (ww + 1) + 1.0 / (zz + 1))
+                    wc, wb = widest(r), widest(ez)
+                    want = (ez, wb) if wb >= r and score(ez, wb) > score(r, wc) else (r, wc)
+                    assert (z, s) == want, (shape, k, j, i, (z, s), want)
+
+
+def test_safe_box_layers():
+    """Flat layers: the box spans the layer's full vertical run and reaches the lateral walls."""
+    rows = np.full((30, 40, 50), 1, dtype=np.uint8)
+    rows[:5] = 2
+    rz, rxy = lib_box(rows)
+    assert rz[2, 20, 25] == 2 and rxy[2, 20, 25] == 19  # epidermis middle: run 2, square to the wall
+    assert rz[0].max() == 0 and rxy[0, 20, 25] == 19  # top layer: no vertical room, full width
+    assert rxy[:, 0].max() == 0 and rxy[:, :, -1].max() == 0

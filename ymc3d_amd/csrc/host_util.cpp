@@ -303,6 +303,115 @@ void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius) {
 // Parametric layered geometry as a voxel grid: the same binary32 cylinder test as the device
 // (each product and sum a separate statement, so nothing is contracted).
 
+// ------------------------------------------------------------------------------------------
+// Safe box of every voxel: half-extents rz (vertical) and rxy (lateral, square) such that every
+// voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-rz, k+rz] has the voxel's row, lies inside the
+// lateral grid and not above the top layer (below the grid, the clamp to the bottom layer).
+// Layered tissue is wide and thin, so a box reaches much further sideways than the cube of
+// safe_radius.  Per voxel:
+//   ez = vertical run of the voxel's row around it (min of the runs above and below, capped);
+//   rxy(t) = largest s such that the s-square of the voxel's slice has the voxel's row everywhere
+//            and ez >= t everywhere, found by binary search over square range-min/max tables;
+// two candidates, (rz, rxy) = (cube radius, rxy(cube radius)) and (ez, rxy(ez)), both of which
+// must contain the cube; of those, the one with the larger harmonic mean of its three
+// half-extents (+1) is kept.
+
+namespace {
+// Square sparse tables of one slice: level m holds min/max over [i, i+2^m) x [j, j+2^m).
+struct SquareTables {
+  int nx, ny, levels;
+  std::vector<uint8_t> ezmin, rowmin, rowmax;  // [levels][ny][nx]
+  void build(const uint8_t *ez, const uint8_t *row, int nx_, int ny_) {
+    nx = nx_;
+    ny = ny_;
+    levels = 1;
+    while ((1 << levels) <= std::max(nx, ny)) ++levels;
+    const size_t plane = (size_t)nx * ny;
+    ezmin.assign(plane * levels, 0);
+    rowmin.assign(plane * levels, 0);
+    rowmax.assign(plane * levels, 0);
+    std::copy(ez, ez + plane, ezmin.begin());
+    std::copy(row, row + plane, rowmin.begin());
+    std::copy(row, row + plane, rowmax.begin());
+    for (int m = 1; m < levels; ++m) {
+      const int h = 1 << (m - 1), w = 1 << m;
+      uint8_t *e = &ezmin[plane * m], *lo = &rowmin[plane * m], *hi = &rowmax[plane * m];
+      const uint8_t *pe = &ezmin[plane * (m - 1)], *plo = &rowmin[plane * (m - 1)], *phi = &rowmax[plane * (m - 1)];
+      for (int j = 0; j + w <= ny; ++j)
+        for (int i = 0; i + w <= nx; ++i) {
+          const size_t a = (size_t)j * nx + i, b = a + h, c = a + (size_t)h * nx, d = c + h;
+          e[a] = std::min(std::min(pe[a], pe[b]), std::min(pe[c], pe[d]));
+          lo[a] = std::min(std::min(plo[a], plo[b]), std::min(plo[c], plo[d]));
+          hi[a] = std::max(std::max(phi[a], phi[b]), std::max(phi[c], phi[d]));
+        }
+    }
+  }
+  // square [i-s, i+s] x [j-s, j+s] (inside the slice): all rows == r and ez >= t
+  bool uniform(int i, int j, int s, uint8_t r, int t) const {
+    const int L = 2 * s + 1;
+    int m = 0;
+    while ((2 << m) <= L) ++m;
+    const int w = 1 << m;
+    const size_t plane = (size_t)nx * ny;
+    const int x0 = i - s, x1 = i + s - w + 1, y0 = j - s, y1 = j + s - w + 1;
+    const size_t q[4] = {(size_t)y0 * nx + x0, (size_t)y0 * nx + x1, (size_t)y1 * nx + x0, (size_t)y1 * nx + x1};
+    for (size_t a : q) {
+      const size_t o = plane * m + a;
+      if (ezmin[o] < t || rowmin[o] != r || rowmax[o] != r) return false;
+    }
+    return true;
+  }
+};
+}  // namespace
+
+void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius, uint8_t *rz, uint8_t *rxy) {
+  const size_t plane = (size_t)nx * ny;
+  std::vector<uint8_t> ez(plane * nz);
+  // vertical runs: up (towards k = 0, stopping at the top layer) and down (clamped below the grid)
+  for (size_t c = 0; c < plane; ++c) {
+    std::vector<int> up(nz), down(nz);
+    for (int k = 0; k < nz; ++k)
+      up[k] = (k > 0 && rows[(size_t)(k - 1) * plane + c] == rows[(size_t)k * plane + c]) ? up[k - 1] + 1 : 0;
+    for (int k = nz - 1; k >= 0; --k)
+      down[k] = (k == nz - 1) ? kSafeMax
+                              : (rows[(size_t)(k + 1) * plane + c] == rows[(size_t)k * plane + c] ? down[k + 1] + 1 : 0);
+    for (int k = 0; k < nz; ++k) ez[(size_t)k * plane + c] = (uint8_t)std::min(std::min(up[k], down[k]), kSafeMax);
+  }
+  unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nthreads; ++t) {
+    pool.emplace_back([&, t] {
+      SquareTables tab;
+      for (int k = (int)t; k < nz; k += (int)nthreads) {
+        const size_t off = (size_t)k * plane;
+        tab.build(&ez[off], &rows[off], nx, ny);
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) {
+            const size_t v = off + (size_t)j * nx + i;
+            const uint8_t r = rows[v];
+            const int smax = std::min(std::min(std::min(i, nx - 1 - i), std::min(j, ny - 1 - j)), kSafeBoxMax);
+            auto widest = [&](int t_min) {
+              if (ez[v] < t_min) return -1;
+              int lo = 0, hi = smax;  // uniform(0) holds: the voxel itself
+              while (lo < hi) {
+                const int mid = (lo + hi + 1) / 2;
+                if (tab.uniform(i, j, mid, r, t_min)) lo = mid; else hi = mid - 1;
+              }
+              return lo;
+            };
+            const int zc = radius[v], zb = ez[v];
+            const int xc = widest(zc), xb = widest(zb);
+            auto score = [](int z, int x) { return 1.0 / (2.0 / (x + 1) + 1.0 / (z + 1)); };
+            const bool take_b = xb >= zc && score(zb, xb) > score(zc, xc);
+            rz[v] = (uint8_t)(take_b ? zb : zc);
+            rxy[v] = (uint8_t)(take_b ? xb : xc);
+          }
+      }
+    });
+  }
+  for (auto &th : pool) th.join();
+}
+
 ymc_status layered_voxelize(const ymc_layered *g, int nx, int ny, int nz, uint8_t *rows) {
   if (!g || !rows || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "layered_voxelize: bad arguments");
   const float r2 = g->vessel_r * g->vessel_r;

@@ -12,9 +12,9 @@
 // issued on CDNA4:
 //  - optical properties of every (table, tissue) row sit in LDS with derived constants
 //    (mu_t, mu_a/mu_t, 1/mu_t, HG terms) precomputed on the host;
-//  - the voxel grid is one byte per voxel in global memory (read through L2; row in bits 0-2 and
-//    the safe radius below in bits 3-7, 16-bit words for grids with 8 or more rows), clamped like
-//    the reference's texture (src/mc3d_gpu.cu:416-418);
+//  - the voxel grid is one 16-bit word per voxel in global memory (read through L2: the row and
+//    the safe box below; 32-bit words for grids with 8 or more rows), clamped like the
+//    reference's texture (src/mc3d_gpu.cu:416-418);
 //  - the boundary distance uses cached reciprocals of the direction cosines, updated only
 //    when the direction changes;
 //  - the next voxel index comes from the face test of the DDA instead of three floor divisions
@@ -23,10 +23,11 @@
 //    wave-uniform branch; lanes that only cross a voxel face still advance the stream by the
 //    four draws the reference consumes on every step;
 //  - plain voxel crossings inside a region of one tissue are taken together: each voxel word
-//    carries a safe radius (Chebyshev distance within which all voxels share its tissue and no
-//    mirror wall or top surface is reached), and the part of a flight that stays inside the safe
-//    cube of its anchor voxel is one jump to its last face, with no voxel load and no Fresnel
-//    test, and the stream advanced by the draws of the crossings it replaces.
+//    carries a safe box (vertical and lateral half-extents of a box of voxels that all share its
+//    tissue and lie inside the lateral walls and below the top surface), and the part of a
+//    flight that stays inside the safe box of its anchor voxel is one jump to its last face,
+//    with no voxel load and no Fresnel test, and the stream advanced by the draws of the
+//    crossings it replaces.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -172,9 +173,11 @@ struct RowConst {
 };
 
 struct KArgs {
-  // geometry: voxel words, tissue row in bits 0-7, safe radius in bits 8-15
-  const uint16_t *vox;
-  const uint8_t *vox8;  // packed voxel bytes (row in bits 0-2, safe radius in bits 3-7) or null
+  // geometry: one word per voxel holding its row and safe box (half-extents rz, rxy).  Grids with
+  // fewer than 8 rows: 16-bit words, row in bits 0-2, rz in 3-7, rxy in 8-15 (vox16); otherwise
+  // 32-bit words, row in bits 0-7, rz in 8-15, rxy in 16-23 (vox32).  One of the two is null.
+  const uint16_t *vox16;
+  const uint32_t *vox32;
   int nx, ny, nz;
   float dx, dy, dz, Lx, Ly;
   float idx, idy, idz;  // 1/dx, 1/dy, 1/dz
@@ -211,7 +214,7 @@ struct KArgs {
   unsigned long long n_chains;
   const volatile int *cancel;
   int poll_interval;
-  int jump_on;   // safe-cube flight jumps enabled
+  int jump_on;   // safe-box flight jumps enabled
   int jump_cap;  // most crossings one jump may take
   int skip_off;  // byte offset of the per-lane stream-skip table in dynamic LDS, or -1 (then the
                  // skipped draws are taken one by one)
@@ -235,27 +238,35 @@ struct Packet {
   float iux, iuy, iuz;
   int i, j, k, t;
   uint32_t nstep;  // steps taken by this packet
-  // Safe cube of the anchor voxel (ai, aj, ak) with radius r, stored as oi = r - ai (likewise oj,
-  // ok) and r2 = 2r so that voxel (i, j, k) lies in the cube iff (unsigned)(i + oi) <= r2 on all
-  // three axes.  No cube: offsets of -2^30 and r2 = 0, which no index passes.
+  // Safe box of the anchor voxel (ai, aj, ak) with half-extents rxy (x, y) and rz, stored as
+  // oi = rxy - ai, oj = rxy - aj, ok = rz - ak, rxy2 = 2 rxy, rz2 = 2 rz, so that voxel (i, j, k)
+  // lies in the box iff (unsigned)(i + oi) <= rxy2, (unsigned)(j + oj) <= rxy2 and
+  // (unsigned)(k + ok) <= rz2.  No box: offsets of -2^30 and extents 0, which no index passes.
   int oi, oj, ok;
-  uint32_t r2;
+  uint32_t rxy2, rz2;
 };
 
 constexpr int kNoCube = -(1 << 30);
 
-__device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, int r) {
-  p.oi = r - i;
-  p.oj = r - j;
-  p.ok = r - k;
-  p.r2 = (uint32_t)(2 * r);
+// Voxel word fields after voxel_word / layered_word: row, rz, rxy.
+__device__ __forceinline__ int word_row(uint32_t w) { return (int)(w & 0xffu); }
+__device__ __forceinline__ int word_rz(uint32_t w) { return (int)((w >> 8) & 0xffu); }
+__device__ __forceinline__ int word_rxy(uint32_t w) { return (int)(w >> 16); }
+
+__device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, uint32_t w) {
+  const int rxy = word_rxy(w), rz = word_rz(w);
+  p.oi = rxy - i;
+  p.oj = rxy - j;
+  p.ok = rz - k;
+  p.rxy2 = (uint32_t)(2 * rxy);
+  p.rz2 = (uint32_t)(2 * rz);
 }
 __device__ __forceinline__ void clear_anchor(Packet &p) {
   p.oi = p.oj = p.ok = kNoCube;
-  p.r2 = 0;
+  p.rxy2 = p.rz2 = 0;
 }
-__device__ __forceinline__ bool in_cube(const Packet &p, int i, int j, int k) {
-  return ((uint32_t)(i + p.oi) <= p.r2) & ((uint32_t)(j + p.oj) <= p.r2) & ((uint32_t)(k + p.ok) <= p.r2);
+__device__ __forceinline__ bool in_box(const Packet &p, int i, int j, int k) {
+  return ((uint32_t)(i + p.oi) <= p.rxy2) & ((uint32_t)(j + p.oj) <= p.rxy2) & ((uint32_t)(k + p.ok) <= p.rz2);
 }
 
 // Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
@@ -271,11 +282,11 @@ __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int
   const uint32_t jc = (uint32_t)min(max(j, 0), a.ny - 1);
   const uint32_t kc = (uint32_t)min(max(k, 0), a.nz - 1);
   const uint32_t idx = (kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic;
-  if (a.vox8) {
-    const uint32_t b = a.vox8[idx];
-    return (b & 7u) | ((b >> 3) << 8);
+  if (a.vox16) {
+    const uint32_t w = a.vox16[idx];
+    return (w & 7u) | (((w >> 3) & 31u) << 8) | ((w >> 8) << 16);
   }
-  return a.vox[idx];
+  return a.vox32[idx];
 }
 
 // Index after a move: an axis steps when the new coordinate reached the face ahead (the face at
@@ -307,7 +318,7 @@ __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layer
   }
   r = min(r, min(min(ic, a.nx - 1 - ic), min(jc, a.ny - 1 - jc)));
   r = min(max(r, 0), kSafeMax);
-  return (uint32_t)row | ((uint32_t)r << 8);
+  return (uint32_t)row | ((uint32_t)r << 8) | ((uint32_t)r << 16);  // the cube as a box
 }
 
 template <bool LAYERED>
@@ -344,9 +355,9 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
   p.j = grid_coord(p.y, 0.0f, a.dy);
   p.k = 0;                                         // grid_coord(0, +1) = 0
   const uint32_t w = geometry_word<LAYERED>(a, lg, p.i, p.j, 0);  // uz > 0: never the air override
-  p.t = (int)(w & 0xffu);
+  p.t = word_row(w);
   p.nstep = 0;
-  set_anchor(p, p.i, p.j, 0, (int)(w >> 8));
+  set_anchor(p, p.i, p.j, 0, w);
 }
 
 // Voxel of coordinate c along one axis, clamped to [lo, hi].  Rounding can put it one voxel off
@@ -363,13 +374,13 @@ __device__ __forceinline__ bool axis_ok(float c, int e, float d, bool up) {
   return up ? ((c >= lo) & (c < hi)) : ((c > lo) & (c <= hi));
 }
 
-// Flight jump.  Inside the safe cube of the anchor voxel every voxel has the anchor's tissue, no
+// Flight jump.  Inside the safe box of the anchor voxel every voxel has the anchor's tissue, no
 // mirror wall and no top surface, so each face the straight path crosses there is a plain
 // crossing of the general step: hit, no Fresnel draw, new indices, s -= db*mu_t, four draws.
-// The path stops at the interaction point (optical depth s) or where it leaves the cube,
+// The path stops at the interaction point (optical depth s) or where it leaves the box,
 // whichever comes first.  The jump moves the packet to the last face it crosses before that
 // point, in one step, and advances the stream by the four draws of each crossing it skips; the
-// general step then takes the interaction or the crossing out of the cube.  The draws are skipped
+// general step then takes the interaction or the crossing out of the box.  The draws are skipped
 // with the lane's skip table (up to two modular products) when the kernel has one in LDS.  Positions are the
 // general step's up to rounding (one fma from the start instead of one per crossing), and the
 // draws are the same, so packets stay in lockstep with step-by-step transport.
@@ -383,15 +394,14 @@ template <bool COUNTERS>
 __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float mu_t,
                                             float inv_mu_t, const uint64_t *skip, LaneAcc &acc) {
   const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
-  // cube of the anchor in index space: [lo, lo + r2] on each axis (empty when there is no cube)
-  const int r2 = (int)p.r2;
+  // box of the anchor in index space: [lo, lo + extent] on each axis (empty when there is none)
   const int loi = -p.oi, loj = -p.oj, lok = -p.ok;
-  const int hii = loi + r2, hij = loj + r2, hik = lok + r2;
+  const int hii = loi + (int)p.rxy2, hij = loj + (int)p.rxy2, hik = lok + (int)p.rz2;
   const float tex = ((float)(upx ? hii + 1 : loi) * a.dx - p.x) * p.iux;
   const float tey = ((float)(upy ? hij + 1 : loj) * a.dy - p.y) * p.iuy;
   const float tez = ((float)(upz ? hik + 1 : lok) * a.dz - p.z) * p.iuz;
   const float t_stop = fminf(fminf(tex, fminf(tey, tez)), p.s * inv_mu_t);
-  // voxel at the stop point, clamped into the cube
+  // voxel at the stop point, clamped into the box
   const int ei = axis_voxel(fmaf(p.ux, t_stop, p.x), a.idx, loi, hii);
   const int ej = axis_voxel(fmaf(p.uy, t_stop, p.y), a.idy, loj, hij);
   const int ek = axis_voxel(fmaf(p.uz, t_stop, p.z), a.idz, lok, hik);
@@ -416,7 +426,7 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   // one predicate without short-circuit branches
   const int cap = skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
   const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= cap) &
-                  !on_face & in_cube(p, p.i, p.j, p.k) & axis_ok(qx, ei, a.dx, upx) &
+                  !on_face & in_box(p, p.i, p.j, p.k) & axis_ok(qx, ei, a.dx, upx) &
                   axis_ok(qy, ej, a.dy, upy) & axis_ok(qz, ek, a.dz, upz) & (qx > 0.0f) & (qx < a.Lx) &
                   (qy > 0.0f) & (qy < a.Ly) & (dbm <= p.s) & (p.nstep + (uint32_t)n < kMaxSteps);
   if (ok) {
@@ -493,7 +503,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   }
   const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
-  const int next_t = air_override ? YMC_TISSUE_AIR : (int)(nw & 0xffu);
+  const int next_t = air_override ? YMC_TISSUE_AIR : word_row(nw);
   const float nt = tab[next_t].r0.w;
 
   float nux = p.ux, nuy = p.uy, nuz = p.uz;
@@ -623,7 +633,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     if (air_override)
       clear_anchor(p);
     else
-      set_anchor(p, ni_, nj_, nk_, (int)(nw >> 8));
+      set_anchor(p, ni_, nj_, nk_, nw);
   }
   if (COUNTERS) acc.steps++;
 
@@ -779,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       }
     }
     if (__ballot(alive) == 0) break;
-    // The plain crossings of the flight ahead, inside the anchor's safe cube, in one jump.
+    // The plain crossings of the flight ahead, inside the anchor's safe box, in one jump.
     if (a.jump_on && alive) {
       const float4 r1 = rows[(size_t)table * a.num_types + p.t].r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
       jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, s_skip, acc);
@@ -907,7 +917,7 @@ struct ymc_engine {
   int nx = 0, ny = 0, nz = 0;
   float dx = 0, dy = 0, dz = 0;
   bool have_geom = false;
-  bool vox8 = false;  // voxel grid stored one byte per voxel
+  bool vox16 = false;  // voxel grid stored as 16-bit words (else 32-bit)
   bool last_skip_table = false;  // the last run skipped jumped draws with the LDS table
   // parametric geometry (ymc_engine_set_layered) instead of a voxel grid
   bool layered = false;
@@ -1057,6 +1067,14 @@ ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t
   return YMC_OK;
 }
 
+ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *rz, uint8_t *rxy) {
+  if (!rows || !rz || !rxy || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "ymc_safe_box: bad arguments");
+  std::vector<uint8_t> radius((size_t)nx * ny * nz);
+  safe_radius(rows, nx, ny, nz, radius.data());
+  safe_box(rows, nx, ny, nz, radius.data(), rz, rxy);
+  return YMC_OK;
+}
+
 ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *out) {
   if (!rows || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "ymc_safe_radius: bad arguments");
   safe_radius(rows, nx, ny, nz, out);
@@ -1070,21 +1088,22 @@ ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
     return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: invalid geometry");
   size_t nv = (size_t)g->nx * g->ny * g->nz;
   if (nv >= (1ull << 31)) return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: more than 2^31 voxels");
-  std::vector<uint8_t> radius(nv);
+  std::vector<uint8_t> radius(nv), rz(nv), rxy(nv);
   safe_radius(g->tissue, g->nx, g->ny, g->nz, radius.data());
+  safe_box(g->tissue, g->nx, g->ny, g->nz, radius.data(), rz.data(), rxy.data());
   uint8_t max_row = 0;
   for (size_t v = 0; v < nv; ++v) max_row = std::max(max_row, g->tissue[v]);
-  e->vox8 = max_row < 8;  // rows fit 3 bits: one byte per voxel with the safe radius (<= 31) above
-  if (e->vox8) {
-    std::vector<uint8_t> bytes(nv);
-    for (size_t v = 0; v < nv; ++v) bytes[v] = (uint8_t)(g->tissue[v] | (std::min<int>(radius[v], 31) << 3));
-    if ((st = e->vox.ensure(nv))) return st;
-    HIP_TRY(hipMemcpy(e->vox.p, bytes.data(), nv, hipMemcpyHostToDevice));
-  } else {
+  e->vox16 = max_row < 8;  // rows fit 3 bits: 16-bit words (rz <= 31 fits 5 bits)
+  if (e->vox16) {
     std::vector<uint16_t> words(nv);
-    for (size_t v = 0; v < nv; ++v) words[v] = (uint16_t)(g->tissue[v] | (radius[v] << 8));
+    for (size_t v = 0; v < nv; ++v) words[v] = (uint16_t)(g->tissue[v] | (rz[v] << 3) | (rxy[v] << 8));
     if ((st = e->vox.ensure(nv * 2))) return st;
     HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 2, hipMemcpyHostToDevice));
+  } else {
+    std::vector<uint32_t> words(nv);
+    for (size_t v = 0; v < nv; ++v) words[v] = (uint32_t)g->tissue[v] | ((uint32_t)rz[v] << 8) | ((uint32_t)rxy[v] << 16);
+    if ((st = e->vox.ensure(nv * 4))) return st;
+    HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 4, hipMemcpyHostToDevice));
   }
   e->nx = g->nx;
   e->ny = g->ny;
@@ -1259,8 +1278,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   if (e->absorbing) HIP_TRY(hipMemsetAsync(e->absorb.p, 0, L * nvox * 8, s));
 
   KArgs a{};
-  a.vox = e->vox.as<uint16_t>();
-  a.vox8 = (e->vox8 && !e->layered) ? e->vox.as<uint8_t>() : nullptr;
+  a.vox16 = (e->vox16 && !e->layered) ? e->vox.as<uint16_t>() : nullptr;
+  a.vox32 = (!e->vox16 && !e->layered) ? e->vox.as<uint32_t>() : nullptr;
   a.layered = e->layered_buf.as<ymc_layered>();
   a.nx = e->nx;
   a.ny = e->ny;
