@@ -280,6 +280,12 @@ ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t 
                                uint32_t n_streams, uint32_t n_draws, int32_t kind, float *out,
                                uint64_t *x_final);
 
+/* Parametric-geometry test hook: the device's voxel word of every voxel of the grid g describes,
+ * out[(k*ny + j)*nx + i] = row | rz << 8 | rxy << 16, where rz and rxy are the half-extents of the
+ * safe box a flight jump may stay in (see ymc_safe_box).  No counterpart in the reference. */
+ymc_status ymc_engine_layered_words(ymc_engine *e, const ymc_layered *g, int32_t nx, int32_t ny, int32_t nz,
+                                    uint32_t *out);
+
 /* Stream-skip test hook: advances n_streams MWC streams (gpumcml_rng.cu:38-46) past the draws of
  * `crossings` plain voxel crossings (4 draws each, crossings <= 63) with the modular skip table a
  * flight jump uses, and returns the final states; they equal those of 4*crossings draws taken one

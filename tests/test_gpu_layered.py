@@ -41,6 +41,32 @@ def test_layered_equals_materialised_grid(engine, multipliers):
         assert abs(a.total_R()[0] - b.total_R()[0]) < 0.3 * a.total_R_se()[0], w
 
 
+def test_layered_safe_boxes(engine):
+    """Device voxel words of parametric geometries against the voxel grid the host voxeliser
+    writes: rows identical, and every voxel's safe box (half-extents rz, rxy) holds that row only,
+    lies inside the lateral walls and below the top surface (brute force on a sample of voxels)."""
+    rng = np.random.default_rng(5)
+    P = configs.sample_skin2_params(4, seed=31)
+    shape = (48, 40, 36)  # (nx, ny, nz): small enough for brute force, with walls nearby
+    nx, ny, nz = shape
+    for w in range(4):
+        _, geo = configs.skin2_tables(P[w:w + 1])
+        g = geo[0].copy()
+        g["vessel_x"] = g["vessel_x"] * nx / configs.SKIN2_SHAPE[0]
+        g["vessel_z"] = min(float(g["vessel_z"]), nz - 4.0)
+        g["top_voxels"] = min(int(g["top_voxels"]), nz // 3)
+        rows = y.layered_voxelize(g, shape)
+        words = engine.layered_words(g, shape)
+        assert np.array_equal((words & 0xFF).astype(np.uint8), rows), w
+        rz, rxy = (words >> 8) & 0xFF, words >> 16
+        for _ in range(3000):
+            k, j, i = rng.integers(0, nz), rng.integers(0, ny), rng.integers(0, nx)
+            z, s_ = int(rz[k, j, i]), int(rxy[k, j, i])
+            assert i - s_ >= 0 and i + s_ <= nx - 1 and j - s_ >= 0 and j + s_ <= ny - 1 and k - z >= 0
+            ks = np.minimum(np.arange(k - z, k + z + 1), nz - 1)
+            assert np.all(rows[ks][:, j - s_:j + s_ + 1, i - s_:i + s_ + 1] == rows[k, j, i]), (w, k, j, i, z, s_)
+
+
 def test_layered_batch_matches_oracle(engine, multipliers):
     P = configs.sample_skin2_params(6, seed=22)
     props, geo = configs.skin2_tables(P)

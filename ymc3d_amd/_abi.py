@@ -143,6 +143,7 @@ _SIGS = {
     "ymc_engine_pool_lanes": (C.c_int, [C.c_void_p, P(C.c_int32)]),
     "ymc_engine_rng_fill": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint32), C.c_uint32, C.c_uint32,
                                       C.c_int32, P(C.c_float), P(C.c_uint64)]),
+    "ymc_engine_layered_words": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_uint32)]),
     "ymc_engine_rng_skip": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint32), C.c_uint32, C.c_uint32,
                                       P(C.c_uint64)]),
     "ymc_run_3dmc_gpu": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,

@@ -241,7 +241,7 @@ struct Packet {
   // Safe box of the anchor voxel (ai, aj, ak) with half-extents rxy (x, y) and rz, stored as
   // oi = rxy - ai, oj = rxy - aj, ok = rz - ak, rxy2 = 2 rxy, rz2 = 2 rz, so that voxel (i, j, k)
   // lies in the box iff (unsigned)(i + oi) <= rxy2, (unsigned)(j + oj) <= rxy2 and
-  // (unsigned)(k + ok) <= rz2.  No box: offsets of -2^30 and extents 0, which no index passes.
+  // (unsigned)(k + ok) <= rz2.  No box: offsets of -2^30, which no index passes (extents <= 510).
   int oi, oj, ok;
   uint32_t rxy2, rz2;
 };
@@ -262,8 +262,7 @@ __device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, uint3
   p.rz2 = (uint32_t)(2 * rz);
 }
 __device__ __forceinline__ void clear_anchor(Packet &p) {
-  p.oi = p.oj = p.ok = kNoCube;
-  p.rxy2 = p.rz2 = 0;
+  p.oi = p.oj = p.ok = kNoCube;  // no index passes, whatever the extents hold
 }
 __device__ __forceinline__ bool in_box(const Packet &p, int i, int j, int k) {
   return ((uint32_t)(i + p.oi) <= p.rxy2) & ((uint32_t)(j + p.oj) <= p.rxy2) & ((uint32_t)(k + p.ok) <= p.rz2);
@@ -297,9 +296,11 @@ __device__ __forceinline__ int next_index(int i, float u, float n, float b) {
 }
 
 // Voxel word of a parametric geometry (ymc_layered): the row of voxel (i, j, k) of the grid the
-// parameters describe (same clamp as the texture), and a safe radius bounded from the distances
-// to the layer interface, the cylinder surface and the walls.  The cylinder test is evaluated
-// with every binary32 operation rounded, exactly as the host voxeliser does.
+// parameters describe (same clamp as the texture), and a safe cube as the box, bounded from the
+// distances to the layer interface, the cylinder surface and the walls.  The cylinder test is
+// evaluated with every binary32 operation rounded, exactly as the host voxeliser does.  (Boxes
+// reaching further sideways, beside or above the cylinder, measured no faster on these
+// geometries, whose long walks stay deep in the bottom layer where the cube is already large.)
 __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layered &g, int i, int j, int k) {
   const int ic = min(max(i, 0), a.nx - 1);
   const int jc = min(max(j, 0), a.ny - 1);
@@ -318,7 +319,7 @@ __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layer
   }
   r = min(r, min(min(ic, a.nx - 1 - ic), min(jc, a.ny - 1 - jc)));
   r = min(max(r, 0), kSafeMax);
-  return (uint32_t)row | ((uint32_t)r << 8) | ((uint32_t)r << 16);  // the cube as a box
+  return (uint32_t)row | ((uint32_t)r << 8) | ((uint32_t)r << 16);
 }
 
 template <bool LAYERED>
@@ -838,6 +839,17 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       if (v) atomicAdd(&a.tot[q], v);
     }
   }
+}
+
+// Parametric-geometry test hook: the voxel word (row, rz, rxy) of every voxel, as the transport
+// kernel computes it.
+__global__ void layered_words_kernel(KArgs a, ymc_layered g, uint32_t *out) {
+  const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int i = (int)(v % (uint32_t)a.nx), j = (int)((v / (uint32_t)a.nx) % (uint32_t)a.ny);
+  const int k = (int)(v / ((uint32_t)a.nx * a.ny));
+  out[v] = layered_word(a, g, i, j, k);
 }
 
 // Stream-skip test hook: each lane builds its skip table in LDS, as the transport kernel does at
@@ -1452,6 +1464,27 @@ ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t 
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(out, dout.p, (size_t)n_streams * n_draws * 4, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(x_final, dxf.p, (size_t)n_streams * 8, hipMemcpyDeviceToHost));
+  return YMC_OK;
+}
+
+ymc_status ymc_engine_layered_words(ymc_engine *e, const ymc_layered *g, int32_t nx, int32_t ny, int32_t nz,
+                                    uint32_t *out) {
+  ymc_status st = check_engine(e);
+  if (st) return st;
+  if (!g || !out || nx <= 0 || ny <= 0 || nz <= 0 || (size_t)nx * ny * nz >= (1ull << 31))
+    return fail(YMC_ERR_ARG, "ymc_engine_layered_words: bad arguments");
+  const size_t n = (size_t)nx * ny * nz;
+  DevBuf dout;
+  if ((st = dout.ensure(n * 4))) return st;
+  KArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  hipLaunchKernelGGL(layered_words_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, e->stream, a, *g,
+                     dout.as<uint32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipMemcpy(out, dout.p, n * 4, hipMemcpyDeviceToHost));
   return YMC_OK;
 }
 

@@ -327,6 +327,15 @@ class Engine:
                                             ptr(out, C.c_float), ptr(xf, C.c_uint64)), "ymc_engine_rng_fill")
         return out, xf
 
+    def layered_words(self, geom, shape: tuple[int, int, int]) -> np.ndarray:
+        """Device voxel words [nz, ny, nx] of one parametric geometry: row | rz << 8 | rxy << 16."""
+        g = np.ascontiguousarray(np.atleast_1d(geom)[:1], dtype=_abi.LAYERED_DTYPE)
+        nx, ny, nz = shape
+        out = np.zeros((nz, ny, nx), dtype=np.uint32)
+        check(self._lib.ymc_engine_layered_words(self._h, g.ctypes.data_as(C.c_void_p), nx, ny, nz,
+                                                 ptr(out, C.c_uint32)), "ymc_engine_layered_words")
+        return out
+
     def rng_skip(self, x: np.ndarray, a: np.ndarray, crossings: int) -> np.ndarray:
         """Device stream skip past the draws of `crossings` plain crossings (4 each, <= 63), with the
         modular table a flight jump uses; returns the final states."""
