@@ -222,14 +222,17 @@ ymc_status ymc_layered_voxelize(const ymc_layered *g, int32_t nx, int32_t ny, in
  * reference. */
 ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *out);
 
-/* Safe box per voxel (host): half-extents rz <= 31 (vertical) and rxy <= 255 (lateral, square)
- * such that every voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-rz, k+rz] has the same row, lies
- * inside the lateral grid and not above the top layer (below the grid the bottom layer repeats).
- * Of two candidates, (safe radius, widest square at that height) and (full vertical run, widest
- * square at that height), those containing the safe-radius cube, the one with the larger harmonic
- * mean of rz+1, rxy+1, rxy+1 is kept.  The engine packs it into its voxel words and a
- * flight jump stays inside it; exported as a test hook.  No counterpart in the reference. */
-ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *rz, uint8_t *rxy);
+/* Safe box per voxel (host): extents zup, zdn <= zcap (above and below) and rxy <= xycap (lateral,
+ * square) such that every voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-zup, k+zdn] has the same
+ * row, lies inside the lateral grid and not above the top layer (below the grid the bottom layer
+ * repeats).  Candidates for (zup, zdn): (r, r), (up, dn), (up, r), (r, dn), with r the safe radius
+ * and up/dn the vertical runs of the row, each with its widest lateral square; of those
+ * containing the safe-radius cube, the one with the largest harmonic mean of rxy+1, rxy+1 and
+ * (zup+zdn)/2+1 is kept, the first listed winning ties.  The engine packs it into its voxel words
+ * (caps 15/31 for 16-bit words, 255/255 for 32-bit) and a flight jump stays inside it; exported as
+ * a test hook.  No counterpart in the reference. */
+ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, int32_t zcap, int32_t xycap,
+                        uint8_t *zup, uint8_t *zdn, uint8_t *rxy);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Device engine.                                                                             */
@@ -281,8 +284,8 @@ ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t 
                                uint64_t *x_final);
 
 /* Parametric-geometry test hook: the device's voxel word of every voxel of the grid g describes,
- * out[(k*ny + j)*nx + i] = row | rz << 8 | rxy << 16, where rz and rxy are the half-extents of the
- * safe box a flight jump may stay in (see ymc_safe_box).  No counterpart in the reference. */
+ * out[(k*ny + j)*nx + i] = row | zup << 8 | zdn << 16 | rxy << 24, the extents of the safe box a
+ * flight jump may stay in (see ymc_safe_box).  No counterpart in the reference. */
 ymc_status ymc_engine_layered_words(ymc_engine *e, const ymc_layered *g, int32_t nx, int32_t ny, int32_t nz,
                                     uint32_t *out);
 

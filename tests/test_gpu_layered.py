@@ -43,7 +43,7 @@ def test_layered_equals_materialised_grid(engine, multipliers):
 
 def test_layered_safe_boxes(engine):
     """Device voxel words of parametric geometries against the voxel grid the host voxeliser
-    writes: rows identical, and every voxel's safe box (half-extents rz, rxy) holds that row only,
+    writes: rows identical, and every voxel's safe box (extents zup, zdn, rxy) holds that row only,
     lies inside the lateral walls and below the top surface (brute force on a sample of voxels)."""
     rng = np.random.default_rng(5)
     P = configs.sample_skin2_params(4, seed=31)
@@ -58,13 +58,13 @@ def test_layered_safe_boxes(engine):
         rows = y.layered_voxelize(g, shape)
         words = engine.layered_words(g, shape)
         assert np.array_equal((words & 0xFF).astype(np.uint8), rows), w
-        rz, rxy = (words >> 8) & 0xFF, words >> 16
+        zup, zdn, rxy = (words >> 8) & 0xFF, (words >> 16) & 0xFF, words >> 24
         for _ in range(3000):
             k, j, i = rng.integers(0, nz), rng.integers(0, ny), rng.integers(0, nx)
-            z, s_ = int(rz[k, j, i]), int(rxy[k, j, i])
-            assert i - s_ >= 0 and i + s_ <= nx - 1 and j - s_ >= 0 and j + s_ <= ny - 1 and k - z >= 0
-            ks = np.minimum(np.arange(k - z, k + z + 1), nz - 1)
-            assert np.all(rows[ks][:, j - s_:j + s_ + 1, i - s_:i + s_ + 1] == rows[k, j, i]), (w, k, j, i, z, s_)
+            zu, zd, s_ = int(zup[k, j, i]), int(zdn[k, j, i]), int(rxy[k, j, i])
+            assert i - s_ >= 0 and i + s_ <= nx - 1 and j - s_ >= 0 and j + s_ <= ny - 1 and k - zu >= 0
+            ks = np.minimum(np.arange(k - zu, k + zd + 1), nz - 1)
+            assert np.all(rows[ks][:, j - s_:j + s_ + 1, i - s_:i + s_ + 1] == rows[k, j, i]), (w, k, j, i, zu, zd, s_)
 
 
 def test_layered_batch_matches_oracle(engine, multipliers):

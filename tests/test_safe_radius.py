@@ -62,31 +62,44 @@ def test_safe_radius_uniform_block_and_cap():
     assert r[39, 35, 35] == 31  # bottom repeats below the grid
 
 
-def lib_box(rows):
+def lib_box(rows, zcap=255, xycap=255):
     rows = np.ascontiguousarray(rows, dtype=np.uint8)
     nz, ny, nx = rows.shape
-    rz, rxy = np.zeros_like(rows), np.zeros_like(rows)
-    assert y.load_library().ymc_safe_box(ptr(rows, C.c_uint8), nx, ny, nz, ptr(rz, C.c_uint8), ptr(rxy, C.c_uint8)) == 0
-    return rz, rxy
+    zup, zdn, rxy = np.zeros_like(rows), np.zeros_like(rows), np.zeros_like(rows)
+    assert y.load_library().ymc_safe_box(ptr(rows, C.c_uint8), nx, ny, nz, zcap, xycap, ptr(zup, C.c_uint8),
+                                         ptr(zdn, C.c_uint8), ptr(rxy, C.c_uint8)) == 0
+    return zup, zdn, rxy
 
 
-def _box_ok(rows, k, j, i, z, s):
-    """Box [i-s, i+s] x [j-s, j+s] x [k-z, k+z]: voxel's row everywhere, inside the lateral grid,
+def _box_ok(rows, k, j, i, zu, zd, s):
+    """Box [i-s, i+s] x [j-s, j+s] x [k-zu, k+zd]: voxel's row everywhere, inside the lateral grid,
     not above the top layer, bottom layer repeating below the grid."""
     nz, ny, nx = rows.shape
-    if i - s < 0 or i + s > nx - 1 or j - s < 0 or j + s > ny - 1 or k - z < 0:
+    if i - s < 0 or i + s > nx - 1 or j - s < 0 or j + s > ny - 1 or k - zu < 0:
         return False
-    ks = np.minimum(np.arange(k - z, k + z + 1), nz - 1)
+    ks = np.minimum(np.arange(k - zu, k + zd + 1), nz - 1)
     return bool(np.all(rows[ks][:, j - s:j + s + 1, i - s:i + s + 1] == rows[k, j, i]))
+
+
+def _runs(rows, k, j, i, cap):
+    nz = rows.shape[0]
+    up = 0
+    while up < cap and k - up - 1 >= 0 and rows[k - up - 1, j, i] == rows[k, j, i]:
+        up += 1
+    dn = 0
+    while dn < cap and rows[min(k + dn + 1, nz - 1), j, i] == rows[k, j, i]:
+        dn += 1
+    return up, dn
 
 
 def test_safe_box_matches_definition():
     """Brute force on the definition: each voxel's box holds its row only and stays in bounds; it
-    contains the safe-radius cube; its lateral extent is the widest square at its height; and it is
-    the better, by the harmonic mean of rz+1, rxy+1, rxy+1, of the two candidate heights (the safe
-    radius and the full vertical run of the row) whose box contains the cube."""
+    contains the safe-radius cube; its lateral extent is the widest square for its vertical
+    extents; and it is the best, by the harmonic mean of rxy+1, rxy+1 and (zup+zdn)/2+1, of the
+    candidates (r, r), (up, dn), (up, r), (r, dn) that contain the cube (r the safe radius, up/dn
+    the vertical runs of the row), with both storage caps."""
     rng = np.random.default_rng(1)
-    for shape in ((9, 11, 13), (16, 12, 10), (12, 9, 21)):
+    for shape, (zcap, xycap) in (((9, 11, 13), (255, 255)), ((16, 12, 10), (15, 31)), ((12, 9, 21), (2, 3))):
         rows = np.ones(shape, dtype=np.uint8)
         nz = shape[0]
         rows[: rng.integers(1, 4)] = 3  # a top layer
@@ -94,37 +107,44 @@ def test_safe_box_matches_definition():
             lo = [rng.integers(0, s) for s in shape]
             hi = [min(s, l + rng.integers(1, 4)) for s, l in zip(shape, lo)]
             rows[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = rng.integers(0, 4)
-        rz, rxy = lib_box(rows)
+        zup, zdn, rxy = lib_box(rows, zcap, xycap)
         cube = brute(rows)
         for k in range(shape[0]):
             for j in range(shape[1]):
                 for i in range(shape[2]):
-                    z, s, r = int(rz[k, j, i]), int(rxy[k, j, i]), int(cube[k, j, i])
-                    assert _box_ok(rows, k, j, i, z, s), (shape, k, j, i, z, s)
-                    assert z >= r and s >= r
-                    # full vertical run of the row around the voxel (capped at 31)
-                    ez = 0
-                    while ez < 31 and k - ez - 1 >= 0 and rows[k - ez - 1, j, i] == rows[k, j, i] and \
-                            rows[min(k + ez + 1, nz - 1), j, i] == rows[k, j, i]:
-                        ez += 1
+                    zu, zd, s = int(zup[k, j, i]), int(zdn[k, j, i]), int(rxy[k, j, i])
+                    r = min(int(cube[k, j, i]), zcap)
+                    assert _box_ok(rows, k, j, i, zu, zd, s), (shape, k, j, i, zu, zd, s)
+                    assert zu >= r and zd >= r and s >= r and zu <= zcap and zd <= zcap and s <= xycap
+                    up, dn = _runs(rows, k, j, i, zcap)
 
-                    def widest(zz):
+                    def widest(a, b):
                         w = 0
-                        while w < 255 and _box_ok(rows, k, j, i, zz, w + 1):
+                        while w < xycap and _box_ok(rows, k, j, i, a, b, w + 1):
                             w += 1
                         return w
 
-                    score = lambda zz, ww: 1.0 / (2.0 / (ww + 1) + 1.0 / (zz + 1))
-                    wc, wb = widest(r), widest(ez)
-                    want = (ez, wb) if wb >= r and score(ez, wb) > score(r, wc) else (r, wc)
-                    assert (z, s) == want, (shape, k, j, i, (z, s), want)
+                    score = lambda a, b, w: 1.0 / (2.0 / (w + 1) + 2.0 / (a + b + 2))
+                    want = (r, r, widest(r, r))
+                    best = score(*want)
+                    for a, b in ((up, dn), (up, r), (r, dn)):
+                        if (a, b) == want[:2]:
+                            continue
+                        w = widest(a, b)
+                        if w >= r and score(a, b, w) > best:
+                            want, best = (a, b, w), score(a, b, w)
+                    assert (zu, zd, s) == want, (shape, k, j, i, (zu, zd, s), want)
 
 
 def test_safe_box_layers():
-    """Flat layers: the box spans the layer's full vertical run and reaches the lateral walls."""
+    """Flat layers: the box spans the layer's full height above and below the voxel and reaches the
+    lateral walls; the 16-bit storage caps hold."""
     rows = np.full((30, 40, 50), 1, dtype=np.uint8)
     rows[:5] = 2
-    rz, rxy = lib_box(rows)
-    assert rz[2, 20, 25] == 2 and rxy[2, 20, 25] == 19  # epidermis middle: run 2, square to the wall
-    assert rz[0].max() == 0 and rxy[0, 20, 25] == 19  # top layer: no vertical room, full width
+    zup, zdn, rxy = lib_box(rows)
+    assert (zup[2, 20, 25], zdn[2, 20, 25], rxy[2, 20, 25]) == (2, 2, 19)  # epidermis middle
+    assert (zup[0, 20, 25], zdn[0, 20, 25], rxy[0, 20, 25]) == (0, 4, 19)  # top voxel: whole layer below
+    assert (zup[4, 20, 25], zdn[4, 20, 25]) == (4, 0)  # bottom voxel of the layer
     assert rxy[:, 0].max() == 0 and rxy[:, :, -1].max() == 0
+    zup, zdn, rxy = lib_box(rows, 15, 31)
+    assert zup.max() <= 15 and zdn.max() <= 15 and rxy.max() <= 31

@@ -124,7 +124,8 @@ _SIGS = {
                                      C.c_double, C.c_double, P(C.c_double)]),
     "ymc_free": (None, [C.c_void_p]),
     "ymc_safe_radius": (C.c_int, [P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8)]),
-    "ymc_safe_box": (C.c_int, [P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8), P(C.c_uint8)]),
+    "ymc_safe_box": (C.c_int, [P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8),
+                               P(C.c_uint8), P(C.c_uint8)]),
     "ymc_layered_voxelize": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_uint8)]),
     "ymc_engine_set_layered": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float,
                                          C.c_float, C.c_void_p, C.c_int32]),

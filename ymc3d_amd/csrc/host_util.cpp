@@ -304,50 +304,54 @@ void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius) {
 // (each product and sum a separate statement, so nothing is contracted).
 
 // ------------------------------------------------------------------------------------------
-// Safe box of every voxel: half-extents rz (vertical) and rxy (lateral, square) such that every
-// voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-rz, k+rz] has the voxel's row, lies inside the
-// lateral grid and not above the top layer (below the grid, the clamp to the bottom layer).
-// Layered tissue is wide and thin, so a box reaches much further sideways than the cube of
-// safe_radius.  Per voxel:
-//   ez = vertical run of the voxel's row around it (min of the runs above and below, capped);
-//   rxy(t) = largest s such that the s-square of the voxel's slice has the voxel's row everywhere
-//            and ez >= t everywhere, found by binary search over square range-min/max tables;
-// two candidates, (rz, rxy) = (cube radius, rxy(cube radius)) and (ez, rxy(ez)), both of which
-// must contain the cube; of those, the one with the larger harmonic mean of its three
-// half-extents (+1) is kept.
+// Safe box of every voxel: extents zup (towards the top), zdn (down) and rxy (lateral, square)
+// such that every voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-zup, k+zdn] has the voxel's row,
+// lies inside the lateral grid and not above the top layer (below the grid, the clamp to the
+// bottom layer).  Layered tissue is wide and thin, so a box reaches much further sideways than the
+// cube of safe_radius, and a voxel at the top or bottom of a layer keeps the layer's height on
+// one side.  Per voxel:
+//   up, dn = vertical runs of the voxel's row above and below it (capped at zcap);
+//   rxy(a, b) = largest s <= xycap such that the s-square of the voxel's slice has the voxel's row
+//               everywhere, up >= a and dn >= b everywhere, found by binary search over square
+//               range-min/max tables;
+// candidates (r, r), (up, dn), (up, r) and (r, dn) for (zup, zdn), r the cube radius capped at
+// zcap, each with its rxy; among those containing the cube (rxy >= r), the one with the largest
+// harmonic mean of rxy+1, rxy+1 and (zup+zdn)/2+1 is kept, the first listed winning ties.
 
 namespace {
 // Square sparse tables of one slice: level m holds min/max over [i, i+2^m) x [j, j+2^m).
 struct SquareTables {
   int nx, ny, levels;
-  std::vector<uint8_t> ezmin, rowmin, rowmax;  // [levels][ny][nx]
-  void build(const uint8_t *ez, const uint8_t *row, int nx_, int ny_) {
+  std::vector<uint8_t> upmin, dnmin, rowmin, rowmax;  // [levels][ny][nx]
+  void build(const uint8_t *up, const uint8_t *dn, const uint8_t *row, int nx_, int ny_) {
     nx = nx_;
     ny = ny_;
     levels = 1;
     while ((1 << levels) <= std::max(nx, ny)) ++levels;
     const size_t plane = (size_t)nx * ny;
-    ezmin.assign(plane * levels, 0);
-    rowmin.assign(plane * levels, 0);
-    rowmax.assign(plane * levels, 0);
-    std::copy(ez, ez + plane, ezmin.begin());
+    for (auto *v : {&upmin, &dnmin, &rowmin, &rowmax}) v->assign(plane * levels, 0);
+    std::copy(up, up + plane, upmin.begin());
+    std::copy(dn, dn + plane, dnmin.begin());
     std::copy(row, row + plane, rowmin.begin());
     std::copy(row, row + plane, rowmax.begin());
     for (int m = 1; m < levels; ++m) {
       const int h = 1 << (m - 1), w = 1 << m;
-      uint8_t *e = &ezmin[plane * m], *lo = &rowmin[plane * m], *hi = &rowmax[plane * m];
-      const uint8_t *pe = &ezmin[plane * (m - 1)], *plo = &rowmin[plane * (m - 1)], *phi = &rowmax[plane * (m - 1)];
+      const size_t o = plane * m, po = plane * (m - 1);
       for (int j = 0; j + w <= ny; ++j)
         for (int i = 0; i + w <= nx; ++i) {
           const size_t a = (size_t)j * nx + i, b = a + h, c = a + (size_t)h * nx, d = c + h;
-          e[a] = std::min(std::min(pe[a], pe[b]), std::min(pe[c], pe[d]));
-          lo[a] = std::min(std::min(plo[a], plo[b]), std::min(plo[c], plo[d]));
-          hi[a] = std::max(std::max(phi[a], phi[b]), std::max(phi[c], phi[d]));
+          auto mn = [&](const std::vector<uint8_t> &t) {
+            return std::min(std::min(t[po + a], t[po + b]), std::min(t[po + c], t[po + d]));
+          };
+          upmin[o + a] = mn(upmin);
+          dnmin[o + a] = mn(dnmin);
+          rowmin[o + a] = mn(rowmin);
+          rowmax[o + a] = std::max(std::max(rowmax[po + a], rowmax[po + b]), std::max(rowmax[po + c], rowmax[po + d]));
         }
     }
   }
-  // square [i-s, i+s] x [j-s, j+s] (inside the slice): all rows == r and ez >= t
-  bool uniform(int i, int j, int s, uint8_t r, int t) const {
+  // square [i-s, i+s] x [j-s, j+s] (inside the slice): all rows == r, up >= a and dn >= b
+  bool uniform(int i, int j, int s, uint8_t r, int a, int b) const {
     const int L = 2 * s + 1;
     int m = 0;
     while ((2 << m) <= L) ++m;
@@ -355,27 +359,30 @@ struct SquareTables {
     const size_t plane = (size_t)nx * ny;
     const int x0 = i - s, x1 = i + s - w + 1, y0 = j - s, y1 = j + s - w + 1;
     const size_t q[4] = {(size_t)y0 * nx + x0, (size_t)y0 * nx + x1, (size_t)y1 * nx + x0, (size_t)y1 * nx + x1};
-    for (size_t a : q) {
-      const size_t o = plane * m + a;
-      if (ezmin[o] < t || rowmin[o] != r || rowmax[o] != r) return false;
+    for (size_t c : q) {
+      const size_t o = plane * m + c;
+      if (upmin[o] < a || dnmin[o] < b || rowmin[o] != r || rowmax[o] != r) return false;
     }
     return true;
   }
 };
 }  // namespace
 
-void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius, uint8_t *rz, uint8_t *rxy) {
+void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius, int zcap, int xycap,
+              uint8_t *zup, uint8_t *zdn, uint8_t *rxy) {
   const size_t plane = (size_t)nx * ny;
-  std::vector<uint8_t> ez(plane * nz);
-  // vertical runs: up (towards k = 0, stopping at the top layer) and down (clamped below the grid)
+  std::vector<uint8_t> up(plane * nz), dn(plane * nz);
   for (size_t c = 0; c < plane; ++c) {
-    std::vector<int> up(nz), down(nz);
-    for (int k = 0; k < nz; ++k)
-      up[k] = (k > 0 && rows[(size_t)(k - 1) * plane + c] == rows[(size_t)k * plane + c]) ? up[k - 1] + 1 : 0;
-    for (int k = nz - 1; k >= 0; --k)
-      down[k] = (k == nz - 1) ? kSafeMax
-                              : (rows[(size_t)(k + 1) * plane + c] == rows[(size_t)k * plane + c] ? down[k + 1] + 1 : 0);
-    for (int k = 0; k < nz; ++k) ez[(size_t)k * plane + c] = (uint8_t)std::min(std::min(up[k], down[k]), kSafeMax);
+    int run = 0;
+    for (int k = 0; k < nz; ++k) {
+      run = (k > 0 && rows[(size_t)(k - 1) * plane + c] == rows[(size_t)k * plane + c]) ? run + 1 : 0;
+      up[(size_t)k * plane + c] = (uint8_t)std::min(run, zcap);
+    }
+    run = zcap;  // the bottom layer repeats below the grid
+    for (int k = nz - 1; k >= 0; --k) {
+      if (k < nz - 1) run = rows[(size_t)(k + 1) * plane + c] == rows[(size_t)k * plane + c] ? run + 1 : 0;
+      dn[(size_t)k * plane + c] = (uint8_t)std::min(run, zcap);
+    }
   }
   unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<std::thread> pool;
@@ -384,27 +391,41 @@ void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius
       SquareTables tab;
       for (int k = (int)t; k < nz; k += (int)nthreads) {
         const size_t off = (size_t)k * plane;
-        tab.build(&ez[off], &rows[off], nx, ny);
+        tab.build(&up[off], &dn[off], &rows[off], nx, ny);
         for (int j = 0; j < ny; ++j)
           for (int i = 0; i < nx; ++i) {
             const size_t v = off + (size_t)j * nx + i;
-            const uint8_t r = rows[v];
-            const int smax = std::min(std::min(std::min(i, nx - 1 - i), std::min(j, ny - 1 - j)), kSafeBoxMax);
-            auto widest = [&](int t_min) {
-              if (ez[v] < t_min) return -1;
-              int lo = 0, hi = smax;  // uniform(0) holds: the voxel itself
+            const uint8_t row = rows[v];
+            const int smax = std::min(std::min(std::min(i, nx - 1 - i), std::min(j, ny - 1 - j)), xycap);
+            auto widest = [&](int a, int b) {
+              int lo = 0, hi = smax;  // s = 0 holds: the voxel itself
               while (lo < hi) {
                 const int mid = (lo + hi + 1) / 2;
-                if (tab.uniform(i, j, mid, r, t_min)) lo = mid; else hi = mid - 1;
+                if (tab.uniform(i, j, mid, row, a, b)) lo = mid; else hi = mid - 1;
               }
               return lo;
             };
-            const int zc = radius[v], zb = ez[v];
-            const int xc = widest(zc), xb = widest(zb);
-            auto score = [](int z, int x) { return 1.0 / (2.0 / (x + 1) + 1.0 / (z + 1)); };
-            const bool take_b = xb >= zc && score(zb, xb) > score(zc, xc);
-            rz[v] = (uint8_t)(take_b ? zb : zc);
-            rxy[v] = (uint8_t)(take_b ? xb : xc);
+            const int r = std::min<int>(radius[v], zcap), u = up[v], d = dn[v];
+            const int cand[4][2] = {{r, r}, {u, d}, {u, r}, {r, d}};
+            int bu = r, bd = r, bx = widest(r, r);
+            auto score = [](int a, int b, int x) { return 1.0 / (2.0 / (x + 1) + 2.0 / (a + b + 2)); };
+            double best = score(bu, bd, bx);
+            for (int q = 1; q < 4; ++q) {
+              const int a = cand[q][0], b = cand[q][1];
+              if (a == bu && b == bd) continue;
+              const int x = widest(a, b);
+              if (x < r) continue;
+              const double sc = score(a, b, x);
+              if (sc > best) {
+                best = sc;
+                bu = a;
+                bd = b;
+                bx = x;
+              }
+            }
+            zup[v] = (uint8_t)bu;
+            zdn[v] = (uint8_t)bd;
+            rxy[v] = (uint8_t)bx;
           }
       }
     });

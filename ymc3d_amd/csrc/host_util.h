@@ -11,10 +11,10 @@ const char *last_error();
 // Largest safe radius stored per voxel (see safe_radius).
 constexpr int kSafeMax = 31;
 void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius);
-// Safe box of every voxel (see safe_box): vertical half-extent rz <= kSafeMax and lateral
-// half-extent rxy <= kSafeBoxMax, from the cube radius computed by safe_radius.
-constexpr int kSafeBoxMax = 255;
-void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius, uint8_t *rz, uint8_t *rxy);
+// Safe box of every voxel (see safe_box): extents zup, zdn <= zcap above and below the voxel and
+// lateral half-extent rxy <= xycap, from the cube radius computed by safe_radius.
+void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius, int zcap, int xycap,
+              uint8_t *zup, uint8_t *zdn, uint8_t *rxy);
 
 ymc_status layered_voxelize(const ymc_layered *g, int nx, int ny, int nz, uint8_t *rows);
 ymc_status generate_multipliers(uint32_t count, uint32_t *out);

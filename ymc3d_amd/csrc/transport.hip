@@ -173,9 +173,10 @@ struct RowConst {
 };
 
 struct KArgs {
-  // geometry: one word per voxel holding its row and safe box (half-extents rz, rxy).  Grids with
-  // fewer than 8 rows: 16-bit words, row in bits 0-2, rz in 3-7, rxy in 8-15 (vox16); otherwise
-  // 32-bit words, row in bits 0-7, rz in 8-15, rxy in 16-23 (vox32).  One of the two is null.
+  // geometry: one word per voxel holding its row and safe box (extents zup, zdn, rxy).  Grids with
+  // fewer than 8 rows: 16-bit words, row in bits 0-2, zup in 3-6, zdn in 7-10, rxy in 11-15
+  // (vox16); otherwise 32-bit words, one byte each for row, zup, zdn, rxy (vox32).  One of the
+  // two is null.
   const uint16_t *vox16;
   const uint32_t *vox32;
   int nx, ny, nz;
@@ -238,28 +239,27 @@ struct Packet {
   float iux, iuy, iuz;
   int i, j, k, t;
   uint32_t nstep;  // steps taken by this packet
-  // Safe box of the anchor voxel (ai, aj, ak) with half-extents rxy (x, y) and rz, stored as
-  // oi = rxy - ai, oj = rxy - aj, ok = rz - ak, rxy2 = 2 rxy, rz2 = 2 rz, so that voxel (i, j, k)
-  // lies in the box iff (unsigned)(i + oi) <= rxy2, (unsigned)(j + oj) <= rxy2 and
-  // (unsigned)(k + ok) <= rz2.  No box: offsets of -2^30, which no index passes (extents <= 510).
+  // Safe box of the anchor voxel (ai, aj, ak): [ai-rxy, ai+rxy] x [aj-rxy, aj+rxy] x
+  // [ak-zup, ak+zdn], stored as oi = rxy - ai, oj = rxy - aj, ok = zup - ak, rxy2 = 2 rxy,
+  // rz2 = zup + zdn, so that voxel (i, j, k) lies in the box iff (unsigned)(i + oi) <= rxy2,
+  // (unsigned)(j + oj) <= rxy2 and (unsigned)(k + ok) <= rz2.  No box: offsets of -2^30, which no
+  // index passes (extents <= 510).
   int oi, oj, ok;
   uint32_t rxy2, rz2;
 };
 
 constexpr int kNoCube = -(1 << 30);
 
-// Voxel word fields after voxel_word / layered_word: row, rz, rxy.
+// Voxel word fields after voxel_word / layered_word: row | zup << 8 | zdn << 16 | rxy << 24.
 __device__ __forceinline__ int word_row(uint32_t w) { return (int)(w & 0xffu); }
-__device__ __forceinline__ int word_rz(uint32_t w) { return (int)((w >> 8) & 0xffu); }
-__device__ __forceinline__ int word_rxy(uint32_t w) { return (int)(w >> 16); }
 
 __device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, uint32_t w) {
-  const int rxy = word_rxy(w), rz = word_rz(w);
+  const int zup = (int)((w >> 8) & 0xffu), zdn = (int)((w >> 16) & 0xffu), rxy = (int)(w >> 24);
   p.oi = rxy - i;
   p.oj = rxy - j;
-  p.ok = rz - k;
+  p.ok = zup - k;
   p.rxy2 = (uint32_t)(2 * rxy);
-  p.rz2 = (uint32_t)(2 * rz);
+  p.rz2 = (uint32_t)(zup + zdn);
 }
 __device__ __forceinline__ void clear_anchor(Packet &p) {
   p.oi = p.oj = p.ok = kNoCube;  // no index passes, whatever the extents hold
@@ -283,7 +283,7 @@ __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int
   const uint32_t idx = (kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic;
   if (a.vox16) {
     const uint32_t w = a.vox16[idx];
-    return (w & 7u) | (((w >> 3) & 31u) << 8) | ((w >> 8) << 16);
+    return (w & 7u) | (((w >> 3) & 15u) << 8) | (((w >> 7) & 15u) << 16) | ((w >> 11) << 24);
   }
   return a.vox32[idx];
 }
@@ -319,7 +319,7 @@ __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layer
   }
   r = min(r, min(min(ic, a.nx - 1 - ic), min(jc, a.ny - 1 - jc)));
   r = min(max(r, 0), kSafeMax);
-  return (uint32_t)row | ((uint32_t)r << 8) | ((uint32_t)r << 16);
+  return (uint32_t)row | ((uint32_t)r << 8) | ((uint32_t)r << 16) | ((uint32_t)r << 24);
 }
 
 template <bool LAYERED>
@@ -1079,11 +1079,14 @@ ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t
   return YMC_OK;
 }
 
-ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, uint8_t *rz, uint8_t *rxy) {
-  if (!rows || !rz || !rxy || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "ymc_safe_box: bad arguments");
+ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, int32_t zcap, int32_t xycap,
+                        uint8_t *zup, uint8_t *zdn, uint8_t *rxy) {
+  if (!rows || !zup || !zdn || !rxy || nx <= 0 || ny <= 0 || nz <= 0 || zcap < 0 || zcap > 255 || xycap < 0 ||
+      xycap > 255)
+    return fail(YMC_ERR_ARG, "ymc_safe_box: bad arguments");
   std::vector<uint8_t> radius((size_t)nx * ny * nz);
   safe_radius(rows, nx, ny, nz, radius.data());
-  safe_box(rows, nx, ny, nz, radius.data(), rz, rxy);
+  safe_box(rows, nx, ny, nz, radius.data(), zcap, xycap, zup, zdn, rxy);
   return YMC_OK;
 }
 
@@ -1100,20 +1103,23 @@ ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
     return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: invalid geometry");
   size_t nv = (size_t)g->nx * g->ny * g->nz;
   if (nv >= (1ull << 31)) return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: more than 2^31 voxels");
-  std::vector<uint8_t> radius(nv), rz(nv), rxy(nv);
+  std::vector<uint8_t> radius(nv), zup(nv), zdn(nv), rxy(nv);
   safe_radius(g->tissue, g->nx, g->ny, g->nz, radius.data());
-  safe_box(g->tissue, g->nx, g->ny, g->nz, radius.data(), rz.data(), rxy.data());
   uint8_t max_row = 0;
   for (size_t v = 0; v < nv; ++v) max_row = std::max(max_row, g->tissue[v]);
-  e->vox16 = max_row < 8;  // rows fit 3 bits: 16-bit words (rz <= 31 fits 5 bits)
+  e->vox16 = max_row < 8;  // rows fit 3 bits: 16-bit words, boxes capped to fit 4 + 4 + 5 bits
+  safe_box(g->tissue, g->nx, g->ny, g->nz, radius.data(), e->vox16 ? 15 : 255, e->vox16 ? 31 : 255, zup.data(),
+           zdn.data(), rxy.data());
   if (e->vox16) {
     std::vector<uint16_t> words(nv);
-    for (size_t v = 0; v < nv; ++v) words[v] = (uint16_t)(g->tissue[v] | (rz[v] << 3) | (rxy[v] << 8));
+    for (size_t v = 0; v < nv; ++v)
+      words[v] = (uint16_t)(g->tissue[v] | (zup[v] << 3) | (zdn[v] << 7) | (rxy[v] << 11));
     if ((st = e->vox.ensure(nv * 2))) return st;
     HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 2, hipMemcpyHostToDevice));
   } else {
     std::vector<uint32_t> words(nv);
-    for (size_t v = 0; v < nv; ++v) words[v] = (uint32_t)g->tissue[v] | ((uint32_t)rz[v] << 8) | ((uint32_t)rxy[v] << 16);
+    for (size_t v = 0; v < nv; ++v)
+      words[v] = (uint32_t)g->tissue[v] | ((uint32_t)zup[v] << 8) | ((uint32_t)zdn[v] << 16) | ((uint32_t)rxy[v] << 24);
     if ((st = e->vox.ensure(nv * 4))) return st;
     HIP_TRY(hipMemcpy(e->vox.p, words.data(), nv * 4, hipMemcpyHostToDevice));
   }

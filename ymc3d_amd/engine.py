@@ -328,7 +328,7 @@ class Engine:
         return out, xf
 
     def layered_words(self, geom, shape: tuple[int, int, int]) -> np.ndarray:
-        """Device voxel words [nz, ny, nx] of one parametric geometry: row | rz << 8 | rxy << 16."""
+        """Device voxel words [nz, ny, nx] of one parametric geometry: row | zup << 8 | zdn << 16 | rxy << 24."""
         g = np.ascontiguousarray(np.atleast_1d(geom)[:1], dtype=_abi.LAYERED_DTYPE)
         nx, ny, nz = shape
         out = np.zeros((nz, ny, nx), dtype=np.uint32)
