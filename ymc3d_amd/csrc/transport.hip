@@ -240,12 +240,13 @@ struct Packet {
   int i, j, k, t;
   uint32_t nstep;  // steps taken by this packet
   // Safe box of the anchor voxel (ai, aj, ak): [ai-rxy, ai+rxy] x [aj-rxy, aj+rxy] x
-  // [ak-zup, ak+zdn], stored as oi = rxy - ai, oj = rxy - aj, ok = zup - ak, rxy2 = 2 rxy,
-  // rz2 = zup + zdn, so that voxel (i, j, k) lies in the box iff (unsigned)(i + oi) <= rxy2,
-  // (unsigned)(j + oj) <= rxy2 and (unsigned)(k + ok) <= rz2.  No box: offsets of -2^30, which no
-  // index passes (extents <= 510).
+  // [ak-zup, ak+zdn], stored as oi = rxy - ai, oj = rxy - aj, ok = zup - ak and the extents
+  // rxy2 = 2 rxy, rz2 = zup + zdn, so that voxel (i, j, k) lies in the box iff
+  // (unsigned)(i + oi) <= rxy2, (unsigned)(j + oj) <= rxy2 and (unsigned)(k + ok) <= rz2.  No box:
+  // offsets of -2^30, which no index passes (extents <= 510).  The two extents share one word
+  // (rxy2 | rz2 << 16): as two fields the compiler kept them in a private array in scratch.
   int oi, oj, ok;
-  uint32_t rxy2, rz2;
+  uint32_t ext;
 };
 
 constexpr int kNoCube = -(1 << 30);
@@ -258,14 +259,14 @@ __device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, uint3
   p.oi = rxy - i;
   p.oj = rxy - j;
   p.ok = zup - k;
-  p.rxy2 = (uint32_t)(2 * rxy);
-  p.rz2 = (uint32_t)(zup + zdn);
+  p.ext = (uint32_t)(2 * rxy) | ((uint32_t)(zup + zdn) << 16);
 }
 __device__ __forceinline__ void clear_anchor(Packet &p) {
   p.oi = p.oj = p.ok = kNoCube;  // no index passes, whatever the extents hold
 }
 __device__ __forceinline__ bool in_box(const Packet &p, int i, int j, int k) {
-  return ((uint32_t)(i + p.oi) <= p.rxy2) & ((uint32_t)(j + p.oj) <= p.rxy2) & ((uint32_t)(k + p.ok) <= p.rz2);
+  const uint32_t rxy2 = p.ext & 0xffffu, rz2 = p.ext >> 16;
+  return ((uint32_t)(i + p.oi) <= rxy2) & ((uint32_t)(j + p.oj) <= rxy2) & ((uint32_t)(k + p.ok) <= rz2);
 }
 
 // Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
@@ -397,7 +398,8 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
   // box of the anchor in index space: [lo, lo + extent] on each axis (empty when there is none)
   const int loi = -p.oi, loj = -p.oj, lok = -p.ok;
-  const int hii = loi + (int)p.rxy2, hij = loj + (int)p.rxy2, hik = lok + (int)p.rz2;
+  const int rxy2 = (int)(p.ext & 0xffffu), rz2 = (int)(p.ext >> 16);
+  const int hii = loi + rxy2, hij = loj + rxy2, hik = lok + rz2;
   const float tex = ((float)(upx ? hii + 1 : loi) * a.dx - p.x) * p.iux;
   const float tey = ((float)(upy ? hij + 1 : loj) * a.dy - p.y) * p.iuy;
   const float tez = ((float)(upz ? hik + 1 : lok) * a.dz - p.z) * p.iuz;
