@@ -145,7 +145,7 @@ def main() -> int:
             "tables": tables,
             "spectra_per_s": args.steps / elapsed if tables > 1 else None,
             "absorption_grid": bool(args.absorption),
-            "launch_params": {k: params[k] for k in ("pool_lanes", "jump_max_crossings",
+            "launch_params": {k: params[k] for k in ("pool_lanes", "jump_max_crossings", "relaunch_lanes",
                                                       "chain_lanes") if k in params},
             "steps_per_photon": steps / n_photons,
             "fresnel_per_photon": fresnel / n_photons,

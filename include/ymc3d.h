@@ -130,6 +130,11 @@ typedef struct ymc_params {
   int32_t jump_max_crossings;  /* most crossings one jump may take; 0 selects 64.  At most 63 when
                                   the jumped draws are skipped with the per-lane LDS table, which
                                   the engine uses whenever it costs no occupancy */
+  int32_t relaunch_lanes;      /* a lane whose packet ended waits until this many lanes of its wave
+                                  wait (tally, next chain, launch run for all at once); 0 selects 6,
+                                  1 launches at once */
+  int32_t relaunch_wait;       /* ... or until some lane has waited this many loop iterations;
+                                  0 selects 16 */
 } ymc_params;
 
 /* Per-table totals (fixed point, scale YMC_FX_ONE). */

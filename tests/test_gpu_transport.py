@@ -152,14 +152,18 @@ def test_c2_sample_parity(engine, multipliers):
 
 
 def test_pool_size_does_not_change_results(engine, multipliers):
+    """Packets draw from their chains' streams: the pool size, the cancel-poll interval and when
+    a lane relaunches (at once, or batched with others of its wave) change no tally bit."""
     scene = configs.skin_slab_sphere(photons=100_000)
     scene.load(engine)
     base = dict(scene.params, flags=y.FLAG_COUNTERS, chain_lanes=8192)
     a = engine.simulate(**base)
-    b = engine.simulate(**base, pool_lanes=256 * 37, compact_interval=97)
-    for f in ("R_w", "R_w2", "R_n", "T_w", "T_n"):
-        assert np.array_equal(getattr(a, f), getattr(b, f)), f
-    assert a.totals.tobytes() == b.totals.tobytes()
+    for over in (dict(pool_lanes=256 * 37, compact_interval=97), dict(relaunch_lanes=1),
+                 dict(relaunch_lanes=64, relaunch_wait=3)):
+        b = engine.simulate(**base, **over)
+        for f in ("R_w", "R_w2", "R_n", "T_w", "T_n"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), (over, f)
+        assert a.totals.tobytes() == b.totals.tobytes(), over
 
 
 def test_energy_identity(engine, multipliers):

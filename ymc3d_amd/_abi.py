@@ -53,7 +53,8 @@ class Params(C.Structure):
                 ("fresnel_mode", C.c_int32), ("kill_zero_step", C.c_int32),
                 ("chain_lanes", C.c_int32), ("pool_lanes", C.c_int32),
                 ("compact_interval", C.c_int32), ("flags", C.c_uint32),
-                ("jump_mode", C.c_int32), ("jump_max_crossings", C.c_int32)]
+                ("jump_mode", C.c_int32), ("jump_max_crossings", C.c_int32),
+                ("relaunch_lanes", C.c_int32), ("relaunch_wait", C.c_int32)]
 
 
 class Totals(C.Structure):

@@ -2,8 +2,10 @@
 
 Searches, on a sample budget of the actual workload:
   pool_lanes        resident lanes (workgroups x 256), up to full occupancy
-  jump_max_crossings  most plain crossings one safe-cube flight jump may take (the wave runs the
-                    stream advance of its longest jump, so a cap trades long jumps for idle lanes)
+  jump_max_crossings  most plain crossings one safe-box flight jump may take
+  relaunch_lanes    a lane whose packet ended waits until this many lanes of its wave wait, then
+                    the wave tallies and relaunches them together (the phase costs the wave as much
+                    for one lane as for many; waiting lanes do nothing)
   chain_lanes       the compaction interval of the persistent pool.  A lane runs a chain's packets
                     back to back, then returns to the work queue for the next chain; the guided
                     schedule gives the first chains ceil(N / (2 chain_lanes)) packets and shrinks
@@ -12,8 +14,8 @@ Searches, on a sample budget of the actual workload:
                     N is, so chain_lanes is searched as a multiple of the full-occupancy pool: short
                     chains keep lanes busy when packet costs vary between tables, long chains cost
                     fewer queue visits and stream seedings.
-pool_lanes does not change results (tallies are bit-identical for any value, which the GPU tests
-check); jump_max_crossings changes only which crossings are jumped, i.e. float rounding of the
+pool_lanes and relaunch_lanes do not change results (tallies are bit-identical for any value,
+which the GPU tests check for the pool); jump_max_crossings changes only which crossings are jumped, i.e. float rounding of the
 positions, with the same draws; chain_lanes decides which stream each packet draws from, so results change at
 the statistical level and stay reproducible for the stored setting.  The best setting goes to a
 JSON file keyed by device name and workload signature; engine runs pick it up through `tuned(...)`.
@@ -33,6 +35,7 @@ CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.
 SEARCH = {
     "pool_frac": (1.0, 0.5),
     "jump_max_crossings": (16, 64),
+    "relaunch_lanes": (4, 6, 8),
     "chain_factor": (1, 4, 16),
 }
 
@@ -86,9 +89,9 @@ def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, ver
     engine.run()
     full = engine.pool_lanes()
     results = []
-    for pf, jc, cf in itertools.product(*SEARCH.values()):
+    for pf, jc, rl, cf in itertools.product(*SEARCH.values()):
         cand = {"pool_lanes": max(256, int(full * pf) // 256 * 256), "jump_max_crossings": jc,
-                "chain_lanes": int(full * cf)}
+                "relaunch_lanes": rl, "chain_lanes": int(full * cf)}
         engine.prepare(**params, **cand)
         ms = min(engine.run() for _ in range(repeats))
         results.append((ms, cand))

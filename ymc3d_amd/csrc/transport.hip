@@ -100,24 +100,28 @@ __device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t 
   return u;
 }
 
-// Per-lane skip table: entry 3(L-1) + d - 1 (L = 1, 2; d = 1..3) holds 2^-32k * 2^64 mod p for
-// k = 4*d*4^L draws, i.e. the state 4*d*4^L - 2 MWC steps after v = 1.  tab points at the lane's
-// first entry; entries are STRIDE apart (one column per lane of the workgroup).
+// Per-lane skip table: entry 3(L-1) + d - 1 (L = 1, 2; d = 1..3) holds E(d*4^L), where E(n) =
+// 2^-128n * 2^64 mod p skips the draws of n crossings (4n steps) in one mwc_mulmod.  E(4) is the
+// state 14 MWC steps after v = 1 (2^-32*14 = 2^-128*4 * 2^64); the others follow from
+// mwc_mulmod(E(a), E(b)) = E(a + b).  tab points at the lane's first entry; entries are STRIDE
+// apart (one column per lane of the workgroup).
 constexpr int kSkipEntries = 6;
 constexpr int kSkipMax = 63;  // crossings one skip covers (three base-4 digits)
 template <int STRIDE>
 __device__ __forceinline__ void build_skip(uint64_t *tab, uint32_t a) {
-  uint64_t st = 1;
-  int k = 0;
-#pragma unroll
-  for (int L = 1; L < 3; ++L) {
-#pragma unroll
-    for (int d = 1; d <= 3; ++d) {
-      const int want = 4 * d * (1 << (2 * L)) - 2;
-      for (; k < want; ++k) mwc_next(st, a);
-      tab[(3 * (L - 1) + d - 1) * STRIDE] = st;
-    }
-  }
+  uint64_t e4 = 1;
+  for (int k = 0; k < 14; ++k) mwc_next(e4, a);
+  const uint64_t e8 = mwc_mulmod(e4, e4, a);
+  const uint64_t e12 = mwc_mulmod(e8, e4, a);
+  const uint64_t e16 = mwc_mulmod(e8, e8, a);
+  const uint64_t e32 = mwc_mulmod(e16, e16, a);
+  const uint64_t e48 = mwc_mulmod(e32, e16, a);
+  tab[0] = e4;
+  tab[STRIDE] = e8;
+  tab[2 * STRIDE] = e12;
+  tab[3 * STRIDE] = e16;
+  tab[4 * STRIDE] = e32;
+  tab[5 * STRIDE] = e48;
 }
 // Advances the stream by the draws of n plain crossings, 0 <= n <= kSkipMax.
 template <int STRIDE>
@@ -219,6 +223,8 @@ struct KArgs {
   int jump_cap;  // most crossings one jump may take
   int skip_off;  // byte offset of the per-lane stream-skip table in dynamic LDS, or -1 (then the
                  // skipped draws are taken one by one)
+  int relaunch_lanes;  // the launch phase runs when this many lanes of a wave wait for it,
+  int relaunch_wait;   // or after this many iterations with a waiting lane
   ymc_photon_record *records;
   unsigned long long n_records;
   // absorption grid [tables][nz][ny][nx] (YMC_FLAG_ABSORPTION) or null
@@ -459,6 +465,7 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
 
 // Outcome of one step.
 enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
+
 
 // One step of a live packet.
 template <bool COUNTERS, bool LAYERED>
@@ -740,8 +747,33 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
   uint32_t left = 0, table = 0, done_in_chain = 0;
   unsigned long long chain = 0;
   bool alive = false, drained = false, stop = false;
+  bool ended = false;  // the lane's packet has ended and waits for its tally
+  int ended_res = kLive;
+  uint32_t waited = 0;  // loop iterations in which some lane of the wave waited for the launch phase
   LaneAcc acc{0, 0, 0, 0};
   uint32_t iter = 0;
+
+  // A packet that ended: tally, optional record, count.
+  auto finish = [&](int res) {
+    unsigned long long *tt = totals + (size_t)table * kTotFields;
+    tally_packet<RADIAL>(p, a, table, tt);
+    if (RECORD) {
+      const unsigned long long gi = a.chain_first[chain] + done_in_chain;
+      if (gi < a.n_records) {
+        ymc_photon_record r;
+        r.x = p.x;
+        r.y = p.y;
+        r.z = p.z;
+        r.w = p.W;
+        r.steps = p.nstep;
+        r.fate = res == kDead ? 2 : (p.z <= 0.0f ? 0 : 1);
+        a.records[gi] = r;
+      }
+    }
+    --left;
+    ++done_in_chain;
+    acc.photons++;
+  };
 
   for (;;) {
     // Cancel flag (host memory), read by the first lane of the wave every poll_interval steps.
@@ -752,7 +784,21 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       stop = stop || (c != 0);
     }
     ++iter;
-    if (!alive && !stop) {
+    // Launch phase: tally ended packets, take new chains, launch packets.  The phase costs the wave
+    // as much for one lane as for many, so lanes whose packet ended wait (doing nothing) and the
+    // phase runs for several at once: when relaunch_lanes lanes wait, when no lane of the wave is
+    // alive, or after relaunch_wait iterations.  Packets draw from their chains' streams, so the
+    // moment a lane takes one changes no result.
+    const bool wants = ended || (!alive && !stop && !(drained && left == 0));
+    const uint32_t nwant = (uint32_t)__popcll(__ballot(wants));
+    const bool phase = nwant >= (uint32_t)a.relaunch_lanes ||
+                       (nwant > 0 && (__ballot(alive) == 0 || waited >= (uint32_t)a.relaunch_wait));
+    waited = phase ? 0 : waited + (nwant > 0 ? 1 : 0);
+    if (phase && ended) {
+      finish(ended_res);
+      ended = false;
+    }
+    if (phase && !alive && !stop) {
       if (left == 0 && !drained) {
         // Chain finished: flush its counters, then take the next chain.  One atomic per wave.
         if (COUNTERS && acc.photons) {
@@ -801,25 +847,10 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
       const RowConst *tab = rows + (size_t)table * a.num_types;
       unsigned long long *tt = totals + (size_t)table * kTotFields;
       const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, a.layered + table, table);
-      if (res != kLive) {
-        tally_packet<RADIAL>(p, a, table, tt);
-        if (RECORD) {
-          const unsigned long long gi = a.chain_first[chain] + done_in_chain;
-          if (gi < a.n_records) {
-            ymc_photon_record r;
-            r.x = p.x;
-            r.y = p.y;
-            r.z = p.z;
-            r.w = p.W;
-            r.steps = p.nstep;
-            r.fate = res == kDead ? 2 : (p.z <= 0.0f ? 0 : 1);
-            a.records[gi] = r;
-          }
-        }
+      if (res != kLive) {  // tallied in the next launch phase
         alive = false;
-        --left;
-        ++done_in_chain;
-        acc.photons++;
+        ended = true;
+        ended_res = res;
       }
     }
   }
@@ -1194,6 +1225,8 @@ static ymc_status prepare_common(ymc_engine *e, const ymc_params *prm) {
   }
   if (prm->jump_mode < 0 || prm->jump_mode > 2) return fail(YMC_ERR_ARG, "jump_mode must be 0, 1 or 2");
   if (prm->jump_max_crossings < 0) return fail(YMC_ERR_ARG, "jump_max_crossings must be >= 0");
+  if (prm->relaunch_lanes < 0 || prm->relaunch_lanes > 64 || prm->relaunch_wait < 0)
+    return fail(YMC_ERR_ARG, "relaunch_lanes must be 0..64 and relaunch_wait >= 0");
   e->params = *prm;
   e->tnx = prm->tally_nx > 0 ? prm->tally_nx : e->nx;
   e->tny = prm->tally_ny > 0 ? prm->tally_ny : e->ny;
@@ -1350,6 +1383,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
   a.jump_on = prm.jump_mode != 2;
   a.jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : 64;
+  a.relaunch_lanes = prm.relaunch_lanes > 0 ? prm.relaunch_lanes : 6;
+  a.relaunch_wait = prm.relaunch_wait > 0 ? prm.relaunch_wait : 16;
   a.records = e->records.as<ymc_photon_record>();
   a.n_records = e->n_records;
   a.abs_grid = e->absorbing ? e->absorb.as<unsigned long long>() : nullptr;
