@@ -131,7 +131,7 @@ typedef struct ymc_params {
                                   the jumped draws are skipped with the per-lane LDS table, which
                                   the engine uses whenever it costs no occupancy */
   int32_t relaunch_lanes;      /* a lane whose packet ended waits until this many lanes of its wave
-                                  wait (tally, next chain, launch run for all at once); 0 selects 6,
+                                  wait (tally, next chain, launch run for all at once); 0 selects 12,
                                   1 launches at once */
   int32_t relaunch_wait;       /* ... or until some lane has waited this many loop iterations;
                                   0 selects 16 */

@@ -35,7 +35,7 @@ CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.
 SEARCH = {
     "pool_frac": (1.0, 0.5),
     "jump_max_crossings": (16, 64),
-    "relaunch_lanes": (4, 6, 8),
+    "relaunch_lanes": (8, 12, 16),
     "chain_factor": (1, 4, 16),
 }
 

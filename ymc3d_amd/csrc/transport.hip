@@ -1383,7 +1383,7 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
   a.jump_on = prm.jump_mode != 2;
   a.jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : 64;
-  a.relaunch_lanes = prm.relaunch_lanes > 0 ? prm.relaunch_lanes : 6;
+  a.relaunch_lanes = prm.relaunch_lanes > 0 ? prm.relaunch_lanes : 12;
   a.relaunch_wait = prm.relaunch_wait > 0 ? prm.relaunch_wait : 16;
   a.records = e->records.as<ymc_photon_record>();
   a.n_records = e->n_records;
