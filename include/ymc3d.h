@@ -124,7 +124,7 @@ typedef struct ymc_params {
   int32_t pool_lanes;          /* resident lanes (device); 0 selects full occupancy */
   int32_t compact_interval;    /* loop iterations between polls of the cancel flag; 0 selects 1024 */
   uint32_t flags;              /* YMC_FLAG_* */
-  int32_t jump_mode;           /* safe-cube flight jumps: 0 selects default (on), 1 on, 2 off.  A
+  int32_t jump_mode;           /* safe-box flight jumps: 0 selects default (on), 1 on, 2 off.  A
                                   flight whose faces all separate voxels of one tissue moves to
                                   its last face at once, drawing what the plain crossings draw */
   int32_t jump_max_crossings;  /* most crossings one jump may take; 0 selects 64.  At most 63 when

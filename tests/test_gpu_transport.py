@@ -219,7 +219,7 @@ def test_run_3dmc_gpu_dropin(multipliers):
 
 @pytest.mark.parametrize("scene_fn,n", [(configs.skin_slab_sphere, 20000), (configs.skin_vessels, 5000)])
 def test_flight_jumps_keep_lockstep(engine, multipliers, scene_fn, n):
-    """A jump replaces plain crossings inside a safe cube by one move and advances the stream by
+    """A jump replaces plain crossings inside a safe box by one move and advances the stream by
     their draws: with the same stream per packet, transport with and without jumps takes the same
     draws, and packets differ only where a rounding difference in the jumped position flips a
     later comparison.  The bars are those of the device-against-oracle lockstep test."""
