@@ -35,6 +35,27 @@ def test_rng_bit_exact(engine, multipliers):
     assert got.min() > 0.0 and got.max() <= 1.0  # oc in (0, 1]
 
 
+def test_draw_conversion_edges(engine, multipliers):
+    """rand_MWC_co / rand_MWC_oc at the edges of the round-toward-zero conversion: zero, values
+    below 2^24 (exact), just above 2^24 and 2^31 (low bits truncated), and values that round to
+    nearest up to 2^32.  A state with low word 0 and carry u draws u first."""
+    us = [0, 1, 2, 255, 0xFFFFFF, 0x1000000, 0x1000001, 0x1FFFFFF, 0x7FFFFFFF, 0x80000000, 0x80000081,
+          0xFFFFFF7F, 0xFFFFFF80, 0xFFFFFFFE, 0xFFFFFFFF]
+    x = np.array([u << 32 for u in us], dtype=np.uint64)
+    a = multipliers[:len(us)].astype(np.uint32)
+
+    def rz(u):  # keep the 24 leading significant bits, then scale by 2^-32 (both exact in f32)
+        sh = max(0, u.bit_length() - 24)
+        return np.float32((u >> sh) << sh) * np.float32(2.0 ** -32)
+
+    co = np.array([rz(u) for u in us], dtype=np.float32)
+    for kind, want in ((0, co), (1, np.float32(1.0) - co)):
+        got, _ = engine.rng_fill(x, a, 1, kind)
+        assert np.array_equal(got[0].view(np.uint32), want.view(np.uint32)), kind
+        ora, _ = O.mwc_fill(x, a, 1, kind)
+        assert np.array_equal(ora[0].view(np.uint32), want.view(np.uint32)), kind
+
+
 def test_stream_skip_bit_exact(engine, multipliers):
     """A flight jump skips the draws of n crossings with three modular products (Montgomery, one
     base-4 digit of n each); the states must equal those of 4n draws taken one by one, for every n a

@@ -58,10 +58,14 @@ __device__ __forceinline__ uint32_t mwc_next(uint64_t &x, uint32_t a) {
 
 // Round-toward-zero u32 -> f32, scaled by 2^-32: rand_MWC_co (gpumcml_rng.cu:41).  Masking the
 // bits below the 24-bit significand makes the conversion exact; the scale is a power of two.
+// With lz leading zeros, the mask 0xffffff00 >> lz (arithmetic shift, amount mod 32) clears the
+// 8 - lz low bits for lz < 8 and nothing otherwise.  v_ffbh_u32 gives lz, and 0xffffffff for
+// u == 0, which shifts the mask to all ones: three instructions instead of __clz's clamp to 32
+// and a variable shift pair.
 __device__ __forceinline__ float u32_co(uint32_t u) {
-  uint32_t lz = __clz(u);                         // 32 for u == 0
-  uint32_t sh = lz < 8u ? 8u - lz : 0u;
-  uint32_t m = (u >> sh) << sh;
+  uint32_t lz;
+  asm("v_ffbh_u32 %0, %1" : "=v"(lz) : "v"(u));
+  const uint32_t m = u & (uint32_t)((int32_t)0xffffff00 >> (lz & 31u));
   return (float)m * 0x1p-32f;
 }
 __device__ __forceinline__ float u32_oc(uint32_t u) { return 1.0f - u32_co(u); }
@@ -183,6 +187,10 @@ struct KArgs {
   // two is null.
   const uint16_t *vox16;
   const uint32_t *vox32;
+  // Field layout of the words the kernel reads (16-bit grid words as above; 32-bit grid words and
+  // parametric-geometry words: bytes): row = w & w_row, zup and zdn are w_zbits wide at bits
+  // w_zup and w_zdn, rxy = w >> w_rxy.
+  uint32_t w_row, w_zup, w_zdn, w_zbits, w_rxy;
   int nx, ny, nz;
   float dx, dy, dz, Lx, Ly;
   float idx, idy, idz;  // 1/dx, 1/dy, 1/dz
@@ -257,18 +265,22 @@ struct Packet {
 
 constexpr int kNoCube = -(1 << 30);
 
-// Voxel word fields after voxel_word / layered_word: row | zup << 8 | zdn << 16 | rxy << 24.
-__device__ __forceinline__ int word_row(uint32_t w) { return (int)(w & 0xffu); }
+// Voxel word fields, in the layout the kernel arguments name (bit-field extracts, no repacking).
+__device__ __forceinline__ int word_row(const KArgs &a, uint32_t w) { return (int)(w & a.w_row); }
 
-__device__ __forceinline__ void set_anchor(Packet &p, int i, int j, int k, uint32_t w) {
-  const int zup = (int)((w >> 8) & 0xffu), zdn = (int)((w >> 16) & 0xffu), rxy = (int)(w >> 24);
-  p.oi = rxy - i;
-  p.oj = rxy - j;
-  p.ok = zup - k;
+// Voxel (i, j, k) with word w becomes the anchor; with box false the packet has no box (offsets
+// that no index passes, whatever the extents hold).  Every field is written on both paths, with
+// selected values: a branch that wrote some fields on one side only led the compiler to keep the
+// box fields in scratch.
+__device__ __forceinline__ void set_anchor(Packet &p, const KArgs &a, int i, int j, int k, uint32_t w,
+                                           bool box = true) {
+  const int zup = (int)__builtin_amdgcn_ubfe(w, a.w_zup, a.w_zbits);
+  const int zdn = (int)__builtin_amdgcn_ubfe(w, a.w_zdn, a.w_zbits);
+  const int rxy = (int)(w >> a.w_rxy);
+  p.oi = box ? rxy - i : kNoCube;
+  p.oj = box ? rxy - j : kNoCube;
+  p.ok = box ? zup - k : kNoCube;
   p.ext = (uint32_t)(2 * rxy) | ((uint32_t)(zup + zdn) << 16);
-}
-__device__ __forceinline__ void clear_anchor(Packet &p) {
-  p.oi = p.oj = p.ok = kNoCube;  // no index passes, whatever the extents hold
 }
 __device__ __forceinline__ bool in_box(const Packet &p, int i, int j, int k) {
   const uint32_t rxy2 = p.ext & 0xffffu, rz2 = p.ext >> 16;
@@ -288,11 +300,7 @@ __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int
   const uint32_t jc = (uint32_t)min(max(j, 0), a.ny - 1);
   const uint32_t kc = (uint32_t)min(max(k, 0), a.nz - 1);
   const uint32_t idx = (kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic;
-  if (a.vox16) {
-    const uint32_t w = a.vox16[idx];
-    return (w & 7u) | (((w >> 3) & 15u) << 8) | (((w >> 7) & 15u) << 16) | ((w >> 11) << 24);
-  }
-  return a.vox32[idx];
+  return a.vox16 ? (uint32_t)a.vox16[idx] : a.vox32[idx];
 }
 
 // Index after a move: an axis steps when the new coordinate reached the face ahead (the face at
@@ -363,9 +371,9 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
   p.j = grid_coord(p.y, 0.0f, a.dy);
   p.k = 0;                                         // grid_coord(0, +1) = 0
   const uint32_t w = geometry_word<LAYERED>(a, lg, p.i, p.j, 0);  // uz > 0: never the air override
-  p.t = word_row(w);
+  p.t = word_row(a, w);
   p.nstep = 0;
-  set_anchor(p, p.i, p.j, 0, w);
+  set_anchor(p, a, p.i, p.j, 0, w);
 }
 
 // Voxel of coordinate c along one axis, clamped to [lo, hi].  Rounding can put it one voxel off
@@ -513,7 +521,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   }
   const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
-  const int next_t = air_override ? YMC_TISSUE_AIR : word_row(nw);
+  const int next_t = air_override ? YMC_TISSUE_AIR : word_row(a, nw);
   const float nt = tab[next_t].r0.w;
 
   float nux = p.ux, nuy = p.uy, nuz = p.uz;
@@ -639,11 +647,8 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.j = nj_;
     p.k = nk_;
     p.t = next_t;
-    // the voxel just loaded becomes the anchor
-    if (air_override)
-      clear_anchor(p);
-    else
-      set_anchor(p, ni_, nj_, nk_, nw);
+    // the voxel just loaded becomes the anchor (no box when the packet is leaving through the top)
+    set_anchor(p, a, ni_, nj_, nk_, nw, !air_override);
   }
   if (COUNTERS) acc.steps++;
 
@@ -1333,6 +1338,12 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   KArgs a{};
   a.vox16 = (e->vox16 && !e->layered) ? e->vox.as<uint16_t>() : nullptr;
   a.vox32 = (!e->vox16 && !e->layered) ? e->vox.as<uint32_t>() : nullptr;
+  const bool w16 = a.vox16 != nullptr;
+  a.w_row = w16 ? 7u : 0xffu;
+  a.w_zup = w16 ? 3u : 8u;
+  a.w_zdn = w16 ? 7u : 16u;
+  a.w_zbits = w16 ? 4u : 8u;
+  a.w_rxy = w16 ? 11u : 24u;
   a.layered = e->layered_buf.as<ymc_layered>();
   a.nx = e->nx;
   a.ny = e->ny;
