@@ -1,0 +1,61 @@
+"""Writes cylinder_edges.json: parametric geometries (vessel axis and radius) with a voxel centre
+whose cylinder test, d2 = fx*fx + fz*fz <= r*r in binary32, gives one answer with every operation
+rounded on its own and the other when either product is fused into an fma with the sum.  The engine
+must give the first answer on the host and the device alike.
+
+Run from the repository root: python tests/golden/make_cylinder_edges.py"""
+import json
+import os
+from fractions import Fraction as Fr
+
+import numpy as np
+
+f32 = np.float32
+
+
+def round32(q: Fr) -> np.float32:
+    """Nearest binary32 to the rational q (ties to even)."""
+    g = f32(float(q))
+    best = g
+    for c in (np.nextafter(g, f32(-np.inf)), g, np.nextafter(g, f32(np.inf))):
+        dc, db = abs(Fr(float(c)) - q), abs(Fr(float(best)) - q)
+        if dc < db or (dc == db and int(c.view(np.uint32)) % 2 == 0):
+            best = c
+    return best
+
+
+def forms(ic, kc, vx, vz):
+    fx = f32(f32(f32(ic) + f32(0.5)) - vx)
+    fz = f32(f32(f32(kc) + f32(0.5)) - vz)
+    ax, az = f32(fx * fx), f32(fz * fz)
+    unfused = f32(ax + az)
+    fused_x = round32(Fr(float(fx)) ** 2 + Fr(float(az)))  # fma(fx, fx, fz*fz)
+    fused_z = round32(Fr(float(fz)) ** 2 + Fr(float(ax)))  # fma(fz, fz, fx*fx)
+    return unfused, fused_x, fused_z
+
+
+def main(n=6, seed=7):
+    rng = np.random.default_rng(seed)
+    cases = []
+    while len(cases) < n:
+        vx, vz, r = f32(rng.uniform(40, 88)), f32(rng.uniform(10, 50)), rng.uniform(3, 12)
+        ang = rng.uniform(0, 2 * np.pi)
+        ic, kc = int(np.floor(float(vx) + r * np.cos(ang))), int(np.floor(float(vz) + r * np.sin(ang)))
+        du, d1, d2 = forms(ic, kc, vx, vz)
+        if du == d1 or du == d2 or (d1 < du) != (d2 < du):
+            continue
+        lo, hi = (min(d1, d2), du) if du > d1 else (du, min(d1, d2))
+        v = f32(np.sqrt(float(lo)))
+        for c in (np.nextafter(v, f32(0)), v, np.nextafter(v, f32(np.inf))):
+            r2 = f32(c * c)
+            if lo <= r2 < hi:
+                cases.append({"ic": ic, "kc": kc, "vessel_x": float(vx), "vessel_z": float(vz), "vessel_r": float(c),
+                              "inside": bool(du <= r2)})
+                break
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cylinder_edges.json")
+    with open(path, "w") as f:
+        json.dump(cases, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

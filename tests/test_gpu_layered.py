@@ -67,6 +67,44 @@ def test_layered_safe_boxes(engine):
             assert np.all(rows[ks][:, j - s_:j + s_ + 1, i - s_:i + s_ + 1] == rows[k, j, i]), (w, k, j, i, zu, zd, s_)
 
 
+def test_cylinder_edges_match_host(engine):
+    """Voxel centres whose cylinder test flips when a product is fused into an fma
+    (tests/golden/cylinder_edges.json): the device's rows equal the host voxeliser's, both in the
+    word hook and inside transport (bit-identical tallies against the materialised grid, jumps off,
+    pencil beam aimed at the edge voxel's column)."""
+    import json
+    import os
+
+    from ymc3d_amd import _abi
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "cylinder_edges.json")) as f:
+        cases = json.load(f)
+    shape = configs.SKIN2_SHAPE
+    P = configs.sample_skin2_params(1, seed=41)
+    props, _ = configs.skin2_tables(P)
+    for n, c in enumerate(cases):
+        g = np.zeros(1, dtype=_abi.LAYERED_DTYPE)
+        g["top_voxels"] = 5
+        g["row_top"], g["row_bottom"], g["row_vessel"] = 1, 2, 3
+        g["vessel_x"], g["vessel_z"], g["vessel_r"] = c["vessel_x"], c["vessel_z"], c["vessel_r"]
+        rows = y.layered_voxelize(g[0], shape)
+        assert (rows[c["kc"], 0, c["ic"]] == 3) == c["inside"], c
+        words = engine.layered_words(g[0], shape)
+        assert np.array_equal((words & 0xFF).astype(np.uint8), rows), c
+        if n < 2:
+            v = configs.SKIN2_VOXEL
+            params = dict(configs.skin2_params(50_000), seed=5, flags=y.FLAG_COUNTERS, jump_mode=2,
+                          pencil_x=(c["ic"] + 0.5) * v)
+            engine.set_layered(g, shape, (v,) * 3)
+            engine.set_optprops(*props)
+            a = engine.simulate(**params)
+            engine.set_geometry(rows, v, v, v)
+            engine.set_optprops(*props)
+            b = engine.simulate(**params)
+            assert a.totals.tobytes() == b.totals.tobytes(), c
+            assert np.array_equal(a.Rr_w, b.Rr_w), c
+
+
 def test_layered_batch_matches_oracle(engine, multipliers):
     P = configs.sample_skin2_params(6, seed=22)
     props, geo = configs.skin2_tables(P)

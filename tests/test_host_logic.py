@@ -45,6 +45,31 @@ def test_layered_voxelize_matches_rule(seed):
     np.testing.assert_array_equal(got, _voxelize_np(g, shape))
 
 
+def _edge_geometry(case):
+    g = np.zeros(1, dtype=_abi.LAYERED_DTYPE)[0]
+    g["top_voxels"] = 5
+    g["row_top"], g["row_bottom"], g["row_vessel"] = 1, 2, 3
+    g["vessel_x"], g["vessel_z"], g["vessel_r"] = case["vessel_x"], case["vessel_z"], case["vessel_r"]
+    return g
+
+
+def test_layered_voxelize_rounds_each_operation():
+    """Voxel centres whose cylinder test flips when a product is fused into an fma with the sum
+    (tests/golden/cylinder_edges.json): the voxeliser gives the answer of separately rounded
+    operations, as does the numpy restatement."""
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "cylinder_edges.json")) as f:
+        cases = json.load(f)
+    assert len(cases) >= 4 and {c["inside"] for c in cases} == {True, False}
+    shape = (128, 2, 64)
+    for c in cases:
+        g = _edge_geometry(c)
+        got = y.layered_voxelize(np.array([g]), shape)
+        assert (got[c["kc"], 0, c["ic"]] == 3) == c["inside"], c
+        np.testing.assert_array_equal(got, _voxelize_np(g, shape))
+
+
 def test_skin2_tables_follow_the_parameter_definitions():
     P = configs.sample_skin2_params(64, 3)
     (mua, mus, g, n), geo = configs.skin2_tables(P)

@@ -300,10 +300,6 @@ void safe_radius(const uint8_t *rows, int nx, int ny, int nz, uint8_t *radius) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Parametric layered geometry as a voxel grid: the same binary32 cylinder test as the device
-// (each product and sum a separate statement, so nothing is contracted).
-
-// ------------------------------------------------------------------------------------------
 // Safe box of every voxel: extents zup (towards the top), zdn (down) and rxy (lateral, square)
 // such that every voxel of [i-rxy, i+rxy] x [j-rxy, j+rxy] x [k-zup, k+zdn] has the voxel's row,
 // lies inside the lateral grid and not above the top layer (below the grid, the clamp to the
@@ -433,7 +429,12 @@ void safe_box(const uint8_t *rows, int nx, int ny, int nz, const uint8_t *radius
   for (auto &th : pool) th.join();
 }
 
+// Parametric layered geometry as a voxel grid: the same binary32 cylinder test as the device, each
+// product and sum rounded on its own (separate statements, contraction off).
 ymc_status layered_voxelize(const ymc_layered *g, int nx, int ny, int nz, uint8_t *rows) {
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
   if (!g || !rows || nx <= 0 || ny <= 0 || nz <= 0) return fail(YMC_ERR_ARG, "layered_voxelize: bad arguments");
   const float r2 = g->vessel_r * g->vessel_r;
   for (int k = 0; k < nz; ++k)

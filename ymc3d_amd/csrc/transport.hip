@@ -70,6 +70,14 @@ __device__ __forceinline__ float u32_co(uint32_t u) {
 }
 __device__ __forceinline__ float u32_oc(uint32_t u) { return 1.0f - u32_co(u); }
 
+// Product rounded on its own.  Device code is compiled with contraction allowed, and HIP's
+// __fmul_rn is a plain operator, so a product feeding a subtraction can become one fma; the pragma
+// keeps this one a separate, rounded multiply.
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 // Stream skip.  An MWC step maps the state v = carry*2^32 + x (0 < v < p, p = a*2^32 - 1) to
 // a*x + carry, which is v*2^-32 mod p (a*2^32 = 1 mod p).  k steps multiply v by 2^-32k mod p, so
 // the draws of n skipped plain crossings (4 each) take one modular product per base-4 digit of n;
@@ -324,10 +332,12 @@ __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layer
   int row = top ? g.row_top : g.row_bottom;
   int r = top ? min(kc, g.top_voxels - 1 - kc) : kc - g.top_voxels;
   if (g.row_vessel >= 0) {
-    const float fx = __fsub_rn(__fadd_rn((float)ic, 0.5f), g.vessel_x);
-    const float fz = __fsub_rn(__fadd_rn((float)kc, 0.5f), g.vessel_z);
-    const float d2 = __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fz, fz));
-    if (d2 <= __fmul_rn(g.vessel_r, g.vessel_r)) row = g.row_vessel;
+    // every operation rounded on its own, as on the host (no contraction into fma)
+#pragma clang fp contract(off)
+    const float fx = ((float)ic + 0.5f) - g.vessel_x;
+    const float fz = ((float)kc + 0.5f) - g.vessel_z;
+    const float d2 = fx * fx + fz * fz;
+    if (d2 <= g.vessel_r * g.vessel_r) row = g.row_vessel;
     // voxels within Chebyshev distance r have centres within r*sqrt(2) in the xz plane
     const float gap = fabsf(sqrtf(d2) - g.vessel_r) - 0.01f;
     r = min(r, gap > 0.0f ? (int)(gap * 0.70710678f) : 0);
@@ -437,7 +447,7 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   const int dj = upy ? ej - p.j : p.j - ej;
   const int dk = upz ? ek - p.k : p.k - ek;
   const int n = di + dj + dk;  // crossings, one step each
-  const float dbm = __fmul_rn(tl, mu_t);
+  const float dbm = mul_rn(tl, mu_t);
   const bool on_face = (p.x == (float)(upx ? p.i + 1 : p.i) * a.dx) | (p.y == (float)(upy ? p.j + 1 : p.j) * a.dy) |
                        (p.z == (float)(upz ? p.k + 1 : p.k) * a.dz);
   // one predicate without short-circuit branches
@@ -479,7 +489,7 @@ enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
 template <bool COUNTERS, bool LAYERED>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
                                             const RowConst *tab, unsigned long long *tot, LaneAcc &acc,
-                                            const ymc_layered *lg, uint32_t table) {
+                                            const ymc_layered *lg, uint32_t table, float4 nrow) {
   const float4 c0 = tab[p.t].r0;
   const float4 c1 = tab[p.t].r1;
   const float ni = c0.w;
@@ -494,7 +504,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const float tz = (bz - p.z) * p.iuz;
   const float db = fminf(tx, fminf(ty, tz));
 
-  const float dbm = __fmul_rn(db, mu_t);  // rounded product (no contraction into s - db*mu_t)
+  const float dbm = mul_rn(db, mu_t);  // rounded product (no contraction into s - db*mu_t)
   const bool hit = dbm <= p.s;                      // :197
   const float ds = hit ? dbm : p.s;                 // :198
   const float hop = hit ? db : ds * c1.z;           // distance moved
@@ -522,7 +532,11 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   const uint32_t nw = geometry_word<LAYERED>(a, lg, ni_, nj_, nk_);
   const bool air_override = nz <= 0.0f && p.uz < 0.0f;
   const int next_t = air_override ? YMC_TISSUE_AIR : word_row(a, nw);
-  const float nt = tab[next_t].r0.w;
+  float nt;  // parametric geometries with up to 4 rows: from the lane's registers
+  if (LAYERED && a.num_types <= 4)
+    nt = next_t == 0 ? nrow.x : (next_t == 1 ? nrow.y : (next_t == 2 ? nrow.z : nrow.w));
+  else
+    nt = tab[next_t].r0.w;
 
   float nux = p.ux, nuy = p.uy, nuz = p.uz;
   bool refl = false;
@@ -757,6 +771,11 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
   uint32_t waited = 0;  // loop iterations in which some lane of the wave waited for the launch phase
   LaneAcc acc{0, 0, 0, 0};
   uint32_t iter = 0;
+  // Parametric geometry of the lane's table and the refractive indices of its rows 0-3, loaded
+  // with each chain: each step otherwise waits on these loads before it can name the next voxel's
+  // row and decide on Fresnel.
+  ymc_layered geo{};
+  float4 nrow = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
 
   // A packet that ended: tally, optional record, count.
   auto finish = [&](int res) {
@@ -831,6 +850,12 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
           table = ch.table;
           rx = a.chain_x[c];
           ra = a.chain_a[c];
+          if (LAYERED) {
+            geo = a.layered[table];
+            const RowConst *tr = a.rows + (size_t)table * a.num_types;
+            const int T = a.num_types;
+            nrow = make_float4(tr[0].r0.w, T > 1 ? tr[1].r0.w : 1.0f, T > 2 ? tr[2].r0.w : 1.0f, T > 3 ? tr[3].r0.w : 1.0f);
+          }
           if (s_skip) build_skip<kBlock>(s_skip, ra);
           done_in_chain = 0;
         } else {
@@ -838,7 +863,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
         }
       }
       if (left > 0) {
-        launch_packet<UNIFORM, LAYERED>(p, rx, ra, a, LAYERED ? a.w0[table] : s_w0[table], a.layered + table);
+        launch_packet<UNIFORM, LAYERED>(p, rx, ra, a, LAYERED ? a.w0[table] : s_w0[table], &geo);
         alive = true;
       }
     }
@@ -851,7 +876,7 @@ __global__ __launch_bounds__(kBlock) void transport_kernel(KArgs a) {
     if (alive) {
       const RowConst *tab = rows + (size_t)table * a.num_types;
       unsigned long long *tt = totals + (size_t)table * kTotFields;
-      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, a.layered + table, table);
+      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, &geo, table, nrow);
       if (res != kLive) {  // tallied in the next launch phase
         alive = false;
         ended = true;
