@@ -59,9 +59,14 @@ def main() -> int:
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the workload, time a plain device-to-device copy of this many bytes (the "
                          "reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes)")
+    ap.add_argument("--atomic-rate", action="store_true",
+                    help="instead of the workload, time 2^27 scattered 64-bit atomic adds into 2^23 counters (the "
+                         "ceiling of the absorption grid's deposits)")
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_reference(args.copy_bytes, max(args.steps, 1))
+    if args.atomic_rate:
+        return atomic_reference(max(args.steps, 1))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the photon budget shards across GPUs with one integer all-reduce of "
@@ -255,6 +260,33 @@ def copy_reference(nbytes: int, reps: int) -> int:
     dt = (time.perf_counter() - t0) / reps
     print(json.dumps({"copy_bytes": nbytes, "launches": reps + 2, "ms_per_copy": dt * 1e3,
                       "GBps_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
+    return 0
+
+
+def atomic_reference(reps: int) -> int:
+    """Random 64-bit atomic adds: torch's index_add_ on int64 issues one device atomic per index.  A
+    64 MiB grid of counters, like the C2 absorption grid, and uniform random indices."""
+    import torch
+
+    nvox, n = 1 << 23, 1 << 27
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    idx = torch.randint(0, nvox, (n,), device="cuda", dtype=torch.int64, generator=gen)
+    ones = torch.ones(n, dtype=torch.int64, device="cuda")
+    grid = torch.zeros(nvox, dtype=torch.int64, device="cuda")
+    grid.index_add_(0, idx, ones)  # warm-up
+    best = float("inf")
+    for _ in range(reps):
+        grid.zero_()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        grid.index_add_(0, idx, ones)
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b))
+    if int(grid.sum()) != n:
+        print("atomic reference: counter sum mismatch", file=sys.stderr)
+        return 1
+    print(json.dumps({"atomic_adds": n, "counters": nvox, "ms": best, "atomics_per_s": n / (best * 1e-3)}), flush=True)
     return 0
 
 
