@@ -47,7 +47,7 @@ PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="skin_vessels", choices=sorted(WORKLOADS))
     ap.add_argument("--photons", type=int, default=0, help="override the workload's packet budget")
