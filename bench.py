@@ -70,7 +70,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the photon budget shards across GPUs with one integer all-reduce of "
-              "the fixed-point tallies, which this build does not run; use --gpus 1", file=sys.stderr)
+              "the fixed-point tallies, which this engine does not run; use --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
