@@ -34,10 +34,11 @@ def forms(ic, kc, vx, vz):
     return unfused, fused_x, fused_z
 
 
-def main(n=6, seed=7):
+def cases(n=6, seed=7):
+    """The first n qualifying geometries drawn from a seeded generator."""
     rng = np.random.default_rng(seed)
-    cases = []
-    while len(cases) < n:
+    found = []
+    while len(found) < n:
         vx, vz, r = f32(rng.uniform(40, 88)), f32(rng.uniform(10, 50)), rng.uniform(3, 12)
         ang = rng.uniform(0, 2 * np.pi)
         ic, kc = int(np.floor(float(vx) + r * np.cos(ang))), int(np.floor(float(vz) + r * np.sin(ang)))
@@ -49,12 +50,16 @@ def main(n=6, seed=7):
         for c in (np.nextafter(v, f32(0)), v, np.nextafter(v, f32(np.inf))):
             r2 = f32(c * c)
             if lo <= r2 < hi:
-                cases.append({"ic": ic, "kc": kc, "vessel_x": float(vx), "vessel_z": float(vz), "vessel_r": float(c),
+                found.append({"ic": ic, "kc": kc, "vessel_x": float(vx), "vessel_z": float(vz), "vessel_r": float(c),
                               "inside": bool(du <= r2)})
                 break
+    return found
+
+
+def main():
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cylinder_edges.json")
     with open(path, "w") as f:
-        json.dump(cases, f, indent=1)
+        json.dump(cases(), f, indent=1)
 
 
 if __name__ == "__main__":

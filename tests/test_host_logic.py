@@ -70,6 +70,28 @@ def test_layered_voxelize_rounds_each_operation():
         np.testing.assert_array_equal(got, _voxelize_np(g, shape))
 
 
+def test_cylinder_edge_fixture_is_current_and_flips_under_fusion():
+    """The fixture equals its generator's output, and in every case the cylinder test with each
+    operation rounded on its own and the test with either product fused into an fma (exact
+    rational arithmetic, rounded once) give opposite answers."""
+    import importlib.util
+    import os
+
+    here = os.path.dirname(__file__)
+    spec = importlib.util.spec_from_file_location("make_cylinder_edges", os.path.join(here, "golden", "make_cylinder_edges.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with open(os.path.join(here, "golden", "cylinder_edges.json")) as f:
+        fixture = json.load(f)
+    assert gen.cases() == fixture
+    f32 = np.float32
+    for c in fixture:
+        unfused, fused_x, fused_z = gen.forms(c["ic"], c["kc"], f32(c["vessel_x"]), f32(c["vessel_z"]))
+        r2 = f32(f32(c["vessel_r"]) * f32(c["vessel_r"]))
+        assert (unfused <= r2) == c["inside"]
+        assert (fused_x <= r2) != c["inside"] and (fused_z <= r2) != c["inside"], c
+
+
 def test_skin2_tables_follow_the_parameter_definitions():
     P = configs.sample_skin2_params(64, 3)
     (mua, mus, g, n), geo = configs.skin2_tables(P)
