@@ -138,3 +138,8 @@ def test_autotune_store(tmp_path, monkeypatch):
     (tmp_path / "tune.json").write_text(json.dumps({"dev": {autotune.signature(a, a.params): {"params": best}}}))
     assert autotune.tuned(b, b.params) == best
     assert autotune.tuned(c, c.params) == {}
+    # the budget's decade is part of the key: 12 packets share it with 10, 1000 do not
+    assert autotune.tuned(configs.skin_slab_sphere(photons=12), a.params | {"num_photons": 12}) == best
+    big = configs.skin_slab_sphere(photons=1000)
+    assert autotune.signature(big, big.params) != autotune.signature(a, a.params)
+    assert autotune.tuned(big, big.params) == {}

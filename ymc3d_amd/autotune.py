@@ -18,7 +18,8 @@ pool_lanes and relaunch_lanes do not change results (tallies are bit-identical f
 which the GPU tests check for the pool); jump_max_crossings changes only which crossings are jumped, i.e. float rounding of the
 positions, with the same draws; chain_lanes decides which stream each packet draws from, so results change at
 the statistical level and stay reproducible for the stored setting.  The best setting goes to a
-JSON file keyed by device name and workload signature; engine runs pick it up through `tuned(...)`.
+JSON file keyed by device name and workload signature (which includes the decade of the packet budget);
+engine runs pick it up through `tuned(...)`.
 """
 from __future__ import annotations
 
@@ -40,13 +41,22 @@ SEARCH = {
 }
 
 
+def budget_decade(params: dict) -> int:
+    """log10 of the packet budget, rounded: the best pool and chains depend on it (a 1e6-packet
+    run is bound by its longest packets, a 1e8-packet run by throughput)."""
+    total = float(np.sum(params["budgets"])) if "budgets" in params else float(params.get("num_photons", 0))
+    return int(np.floor(np.log10(total) + 0.5)) if total > 0 else 0
+
+
 def signature(scene, params: dict) -> str:
-    """Workload key: geometry shape and voxel, table count, illumination and tally layout."""
+    """Workload key: geometry shape and voxel, table count, illumination, tally layout and the decade
+    of the packet budget."""
     h = hashlib.sha1()
     h.update(np.ascontiguousarray(scene.file_values).tobytes())
     h.update(np.ascontiguousarray(scene.records, dtype=np.float64).tobytes())
     for k in ("illumination", "tally_nx", "tally_ny", "radial_bins"):
         h.update(f"{k}={params.get(k, 0)};".encode())
+    h.update(f"budget_decade={budget_decade(params)};".encode())
     return h.hexdigest()[:16]
 
 
