@@ -61,13 +61,22 @@ def signature(scene, params: dict) -> str:
 
 
 def device_name() -> str:
+    """The GPU's marketing name when rocminfo reports one (e.g. "AMD Instinct MI355X"), else its
+    architecture (e.g. "gfx950": some hosts leave the marketing name empty); "gpu" without rocminfo."""
     try:
         import subprocess
 
         out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=30).stdout
+        arch = ""
         for line in out.splitlines():
-            if "Marketing Name" in line and "AMD" in line and "Instinct" in line:
-                return line.split(":", 1)[1].strip()
+            key, _, value = line.strip().partition(":")
+            value = value.strip()
+            if key == "Marketing Name" and "Instinct" in value:
+                return value
+            if key == "Name" and value.startswith("gfx") and not arch:
+                arch = value
+        if arch:
+            return arch
     except Exception:
         pass
     return "gpu"
