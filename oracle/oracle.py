@@ -42,6 +42,9 @@ _SIGS = {
                                        P(C.c_uint64), P(C.c_uint32), C.c_uint64, P(_abi.Tally), C.c_void_p,
                                        C.c_uint64, C.c_int32, C.c_void_p]),
     "ymc_oracle_max_threads": (C.c_int32, []),
+    "ymc_oracle_step": (C.c_int32, [P(_abi.Geometry), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
+                                    C.c_int32, C.c_int32, C.c_uint64, P(C.c_float), P(C.c_int32), P(C.c_uint64),
+                                    C.c_uint32, P(C.c_int32)]),
 }
 
 
@@ -79,6 +82,25 @@ def init_rng(table, seed: int, n: int):
     a = np.empty(n, dtype=np.uint32)
     st = lib().ymc_oracle_init_rng(ptr(t, C.c_uint32), t.size, seed, n, ptr(x, C.c_uint64), ptr(a, C.c_uint32))
     return st, x, a
+
+
+def step(rows, voxel, table, state, idx, x: int, a: int, fresnel_mode: int = 0, kill_zero_step: int = 0,
+         index: int = 0):
+    """One oracle step (src/mc3d_gpu.cu:181-353) of one packet.  rows [nz, ny, nx] uint8; table = (mua, mus,
+    g, n) arrays over the rows; state = (x, y, z, ux, uy, uz, W, s); idx = (i, j, k, t).  Returns (frozen,
+    state', idx', x', fresnel) with x' the MWC state after the step's draws."""
+    r = np.ascontiguousarray(rows, dtype=np.uint8)
+    nz, ny, nx = r.shape
+    geo = _abi.Geometry(nx, ny, nz, voxel[0], voxel[1], voxel[2], ptr(r, C.c_uint8))
+    cols = [np.ascontiguousarray(v, dtype=np.float32) for v in table]
+    st = np.array(state, dtype=np.float32)
+    ix = np.array(idx, dtype=np.int32)
+    xs = np.array([x], dtype=np.uint64)
+    fr = C.c_int32(0)
+    frozen = lib().ymc_oracle_step(C.byref(geo), *(ptr(c, C.c_float) for c in cols), fresnel_mode, kill_zero_step,
+                                   index, ptr(st, C.c_float), ptr(ix, C.c_int32), ptr(xs, C.c_uint64), a,
+                                   C.byref(fr))
+    return bool(frozen), st, ix, int(xs[0]), fr.value
 
 
 def build_chains(budgets, chain_lanes: int):

@@ -14,9 +14,13 @@
 // Parity status of each part (how it is pinned) is listed in tests/test_oracle_*.py headers:
 //   MWC step and float conversion ........ closed form (tests/test_oracle_rng.py)
 //   multipliers .......................... defining primality property, independent checker
-//   init_RNG seeding ..................... invariants + golden vectors
+//   init_RNG seeding ..................... the begin-multiplier stream split into carry and value
+//                                          in Python integers / binary64 (tests/test_oracle_rng.py)
 //   grid coord / intersection ............ brute force on tiny grids
-//   Fresnel / HG / specular .............. closed forms and sampling moments
+//   Fresnel reflectance / HG / specular .. closed forms and sampling moments
+//   Fresnel direction update ............. Snell's law, law of reflection, total internal
+//                                          reflection, one step at a time through ymc_oracle_step
+//                                          (tests/test_oracle_fresnel_direction.py)
 //   transport ............................ van de Hulst slab, energy identity, symmetry
 #include "../include/ymc3d.h"
 
@@ -506,6 +510,52 @@ float ymc_oracle_fresnel(float cos_ai, float ni, float nt, int32_t mode, float *
 
 float ymc_oracle_hg_costheta(float g, float xi_oc) { return hg_costheta(g, xi_oc); }
 float ymc_oracle_specular(float n0, float n1) { return specular(n0, n1); }
+
+// One step of photontrack_step (src/mc3d_gpu.cu:181-353) for one packet on a voxel grid with one
+// optical-property table, exposed so that single events (a Fresnel crossing, an interaction) can
+// be checked against physics on their own.  state[8] = x, y, z, ux, uy, uz, W, s and idx[4] =
+// i, j, k, t (row) are read and written back; *rx is the packet's MWC state.  kill_zero_step and
+// index as for the transport (index = the packet's 0-based step number).  Returns 1 when the
+// packet froze (or was killed), 0 when it lives on; *fresnel receives 1 when the step took the
+// Fresnel branch (:226-257).
+int32_t ymc_oracle_step(const ymc_geometry *geom, const float *mua, const float *mus, const float *g,
+                        const float *n, int32_t fresnel_mode, int32_t kill_zero_step, uint64_t index, float *state,
+                        int32_t *idx, uint64_t *rx, uint32_t ra, int32_t *fresnel) {
+  Geom G{};
+  G.nx = geom->nx;
+  G.ny = geom->ny;
+  G.nz = geom->nz;
+  G.dx = geom->dx;
+  G.dy = geom->dy;
+  G.dz = geom->dz;
+  G.Lx = G.dx * (float)G.nx;
+  G.Ly = G.dy * (float)G.ny;
+  G.t = geom->tissue;
+  Props P{mua, mus, g, n};
+  Photon p{};
+  p.x = state[0];
+  p.y = state[1];
+  p.z = state[2];
+  p.ux = state[3];
+  p.uy = state[4];
+  p.uz = state[5];
+  p.W = state[6];
+  p.s = state[7];
+  p.i = idx[0];
+  p.j = idx[1];
+  p.k = idx[2];
+  p.t = idx[3];
+  Acc acc;
+  const bool froze = step(p, *rx, ra, G, P, fresnel_mode, kill_zero_step, index, acc);
+  const float out[8] = {p.x, p.y, p.z, p.ux, p.uy, p.uz, p.W, p.s};
+  std::memcpy(state, out, sizeof(out));
+  idx[0] = p.i;
+  idx[1] = p.j;
+  idx[2] = p.k;
+  idx[3] = p.t;
+  if (fresnel) *fresnel = (int32_t)acc.fresnel;
+  return froze ? 1 : 0;
+}
 
 // Guided chain schedule, restated from its definition: chain length
 // ceil(remaining_total / (2*chain_lanes)), tables visited round-robin, each chain capped by

@@ -8,10 +8,14 @@ safeprimes_base32.txt is not in the repository):
   1 - co with round-to-nearest (numpy float32 arithmetic).
 - Multipliers: the defining property (a*2^32-1 and a*2^31-1 prime) re-checked by a Python
   Miller-Rabin, plus a gap check that no valid multiplier was skipped at the top of the range.
-- init_RNG (:59-118): the product's seeding must equal the oracle's bit for bit, including the
-  rejection branch (forced with tiny multipliers), and satisfy the state invariants of :83-85.
+- init_RNG (:59-118): every stream is rebuilt in Python from the begin-multiplier MWC stream (the
+  recurrence pinned above): the carry from one output, floored after a binary64 product, the value
+  from the next, and the pairs that break the state invariants of :83-85 drawn again.  Tiny
+  multipliers force the rejections; a seed built to put u*a/2^32 just below an integer separates
+  binary64 rounding from an exact or binary32 floor.  Product and oracle must equal that bit for bit.
 """
 import json
+import math
 import os
 
 import numpy as np
@@ -197,3 +201,101 @@ def test_multiplier_text_reader(tmp_path):
     with pytest.raises(y.YmcError) as ei:
         y.read_multipliers_text(str(tmp_path / "missing.txt"))
     assert ei.value.status == y._abi.ERR_IO
+
+
+# ---------------------------------------------------------------------------------------------
+# init_RNG (gpumcml_rng.cu:88-112) tied to the begin-multiplier MWC stream in Python integers.
+#
+# The begin stream is the MWC recurrence pinned above (multiplier table[0], state = the seed).
+# Stream i takes the next two outputs u, v of that stream: carry c = floor((u / 2^32) * a_i) with
+# the product rounded to the nearest binary64 (the C expression of :106 is evaluated in double;
+# Python floats are IEEE binary64 with round-to-nearest), value x = v (:109-110); the pair is
+# drawn again while the state violates 0 < state, c < a_i - 1, x < 2^32 - 1 (:100).
+
+
+def _begin_outputs(seed: int, begin: int):
+    x = seed
+    while True:
+        x = (x & M32) * begin + (x >> 32)
+        yield x & M32
+
+
+def _py_init_rng(table, seed: int, n: int, carry_rule=None, swap=False):
+    begin = int(table[0])
+    outs = _begin_outputs(seed, begin)
+    carry_rule = carry_rule or (lambda u, a: int(math.floor((float(u) / 4294967296.0) * float(a))))
+    xs, As, rejected = [], [], 0
+    for i in range(n):
+        a = int(table[1 + i % (len(table) - 1)])
+        while True:
+            u, v = next(outs), next(outs)
+            if swap:
+                u, v = v, u
+            c = carry_rule(u, a) & M32
+            state = (c << 32) | v
+            if state != 0 and c < a - 1 and v < M32:
+                break
+            rejected += 1
+        xs.append(state)
+        As.append(a)
+    return np.array(xs, np.uint64), np.array(As, np.uint32), rejected
+
+
+@pytest.mark.parametrize("seed", [1, 12345, (7 << 32) | 99, 2 ** 40 + 17])
+def test_seeding_is_the_begin_stream_split_into_carry_and_value(multipliers, seed):
+    tab = [int(v) for v in multipliers[:257]]
+    want_x, want_a, _ = _py_init_rng(tab, seed, 600)
+    x, a = y.seed_streams(np.array(tab, np.uint32), seed, 600)
+    st, xo, ao = O.init_rng(np.array(tab, np.uint32), seed, 600)
+    assert st == 0
+    assert np.array_equal(xo, want_x) and np.array_equal(ao, want_a)
+    assert np.array_equal(x, want_x) and np.array_equal(a, want_a)
+    # the plausible slips give other states: c and x drawn in the other order, the carry floored
+    # from a binary32 product
+    swapped, _, _ = _py_init_rng(tab, seed, 600, swap=True)
+    single = lambda u, a: int(math.floor(float(np.float32(np.float32(u) / np.float32(4294967296.0)) * np.float32(a))))
+    f32, _, _ = _py_init_rng(tab, seed, 600, carry_rule=single)
+    assert np.count_nonzero(swapped != xo) > 590
+    assert np.count_nonzero(f32 != xo) > 300
+
+
+def test_seeding_rejections_follow_the_begin_stream():
+    """Tiny multipliers reject most pairs (gpumcml_rng.cu:100): the streams must skip exactly those."""
+    tab = [13, 3, 5, 7, 4]
+    for seed in (1, 2, 3, 5, 11):
+        want_x, want_a, rejected = _py_init_rng(tab, seed, 400)
+        assert rejected > 100
+        st, xo, ao = O.init_rng(np.array(tab, np.uint32), seed, 400)
+        x, a = y.seed_streams(np.array(tab, np.uint32), seed, 400)
+        assert st == 0
+        assert np.array_equal(xo, want_x) and np.array_equal(ao, want_a)
+        assert np.array_equal(x, want_x) and np.array_equal(a, want_a)
+
+
+def _seed_with_first_output(u: int, begin: int) -> int:
+    """A valid init_RNG seed whose begin stream's first output is u."""
+    for lo in range(1, 1 << 20):
+        hi = (u - lo * begin) % (1 << 32)
+        if hi < begin - 1:
+            return (hi << 32) | lo
+    raise AssertionError("no seed found")
+
+
+def test_seeding_carry_is_floored_after_binary64_rounding(multipliers):
+    """u * a / 2^32 just below an integer k: the binary64 product of :106 rounds to k, so the carry
+    is k; the exact quotient would floor to k - 1."""
+    begin, a = int(multipliers[0]), int(multipliers[1])
+    assert a % 2 == 1
+    inv = pow(a, -1, 1 << 32)
+    u = ((1 << 32) - 1) * inv % (1 << 32)  # u * a = -1 (mod 2^32)
+    exact = (u * a) >> 32
+    rounded = int(math.floor((float(u) / 4294967296.0) * float(a)))
+    assert rounded == exact + 1 and exact + 1 < a - 1
+    seed = _seed_with_first_output(u, begin)
+    tab = np.array([begin, a], np.uint32)
+    st, xo, _ = O.init_rng(tab, seed, 1)
+    x, _ = y.seed_streams(tab, seed, 1)
+    assert st == 0 and int(x[0]) == int(xo[0])
+    assert int(xo[0]) >> 32 == exact + 1
+    want, _, _ = _py_init_rng([begin, a], seed, 1)
+    assert int(want[0]) == int(xo[0])
