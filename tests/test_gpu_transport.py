@@ -172,6 +172,40 @@ def test_c2_sample_parity(engine, multipliers):
     assert g.totals[0]["fresnel"] > 0
 
 
+def test_c2_parity_independent_streams(engine, multipliers):
+    """Layered skin with vessels (256x256x128), uniform illumination: 1.6e7 packets per side on
+    independent streams (seeds 31 and 4099), so the combined standard error is the noise of the
+    difference.  Total R within 3 SE and within 0.5%; the full-resolution map with at most 1% of
+    bins (>= 20 packets each side) beyond 3 SE; on 64x64 coarse bins (about 470 packets each) the
+    mean of z^2 within 0.1 of 1 (its standard deviation for 4096 normal z is 0.022) and at most 1%
+    of bins beyond 3 SE."""
+    n = 16_000_000
+    scene = configs.skin_vessels(photons=n)
+    scene.load(engine)
+    g = engine.simulate(**{**scene.params, "seed": 31})
+    r = O.run_scene(scene, multipliers, seed=4099)
+    d = g.total_R()[0] - r.total_R()[0]
+    se = np.hypot(g.total_R_se()[0], r.total_R_se()[0])
+    print(f"C2 total R gpu {g.total_R()[0]:.6f} oracle {r.total_R()[0]:.6f} diff {d:.2e} se {se:.2e}", flush=True)
+    assert abs(d) < 3 * se
+    assert abs(d) / r.total_R()[0] < 0.005
+    z, nb = _compare_maps(g, r, min_count=20)
+    print(f"C2 independent: bins {nb}, |z|>3 fraction {np.mean(np.abs(z) > 3):.4f}", flush=True)
+    assert nb > 20000 and np.mean(np.abs(z) > 3) <= 0.01
+
+    def coarse(res):
+        w = res.R_w[0].reshape(64, 4, 64, 4).sum((1, 3)).astype(float) / FX
+        w2 = res.R_w2[0].reshape(64, 4, 64, 4).sum((1, 3)).astype(float) / FX
+        return w / n, np.sqrt(np.maximum(w2 / n - (w / n) ** 2, 0) / n)
+
+    mg, sg = coarse(g)
+    mr, sr = coarse(r)
+    zc = (mg - mr) / np.hypot(sg, sr)
+    print(f"C2 independent 64x64: mean z^2 {np.mean(zc ** 2):.4f}, |z|>3 {np.mean(np.abs(zc) > 3):.4f}", flush=True)
+    assert abs(np.mean(zc ** 2) - 1.0) < 0.1
+    assert np.mean(np.abs(zc) > 3) <= 0.01
+
+
 def test_pool_size_does_not_change_results(engine, multipliers):
     """Packets draw from their chains' streams: the pool size, the cancel-poll interval and when
     a lane relaunches (at once, or batched with others of its wave) change no tally bit."""
@@ -229,13 +263,25 @@ def test_cancel_flag(engine, multipliers):
 
 
 def test_run_3dmc_gpu_dropin(multipliers):
-    scene = configs.skin_slab_sphere()
+    """The drop-in (the arguments of run_3dmc_gpu, src/mc3d_gpu.h:16; uniform illumination as the
+    reference hard-wires, :87-95) against the oracle on the same seed and multiplier table: raw
+    R_tot and T_tot within 3 combined SE (same streams, so this bar is loose), maps that sum to the
+    totals, and the finished count."""
+    n_ph = 200_000
+    scene = configs.skin_slab_sphere(photons=n_ph)
     rows = scene.rows.astype(np.int32)
     mua, mus, g, n = (v[0] for v in scene.optprops())
-    R, Rtot, T, Ttot, nf = y.run_3dmc_gpu(rows, *scene.voxel, mua, mus, g, n, 20000, seed=5, multipliers=multipliers)
-    assert nf == 20000
+    R, Rtot, T, Ttot, nf = y.run_3dmc_gpu(rows, *scene.voxel, mua, mus, g, n, n_ph, seed=5, multipliers=multipliers)
+    assert nf == n_ph
     assert R.shape == (64, 64) and abs(R.sum() - Rtot) < 1e-6 * max(Rtot, 1)
-    assert 0.02 < Rtot / nf < 0.6
+    assert T.shape == (64, 64) and abs(T.sum() - Ttot) <= 1e-6 * max(Ttot, 1)
+    r = O.run_scene(scene, multipliers, seed=5, illumination=y._abi.ILLUM_UNIFORM)
+    t = r.totals[0]
+    for got, w, w2 in ((Rtot, t["R_w"], t["R_w2"]), (Ttot, t["T_w"], t["T_w2"])):
+        mean = float(w) / FX / n_ph
+        se = np.sqrt(max(float(w2) / FX / n_ph - mean ** 2, 0.0) / n_ph)
+        print(f"drop-in {got / n_ph:.6f} oracle {mean:.6f} se {se:.2e}")
+        assert abs(got / n_ph - mean) < 3 * np.sqrt(2) * se + 1e-12
 
 
 @pytest.mark.parametrize("scene_fn,n", [(configs.skin_slab_sphere, 20000), (configs.skin_vessels, 5000)])
