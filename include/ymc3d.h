@@ -343,6 +343,16 @@ ymc_status ymc_mlp_forward(ymc_mlp *n, const float *x, int64_t rows, float *y);
 /* The same inputs through `count` (1..8) networks of identical widths (each <= 256, <= 8 layers) on
  * one device, in one fused launch per 4096 rows: y [count][rows][out]. */
 ymc_status ymc_mlp_forward_ensemble(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y);
+/* The same fused ensemble pass, allocating nothing: no hipMalloc, hipFree or allocating library call.
+ * No counterpart in the reference; it is for callers that cannot allocate device memory (tight serving
+ * loops, code that forbids allocation).  Inputs and outputs are the caller's host buffers as for
+ * ymc_mlp_forward_ensemble and the results are identical.  Rows are staged through a 4 MiB buffer
+ * that the library holds on each device for the life of the process: each chunk of rows takes
+ * in + count * out floats per row, chunks hold up to 4096 rows, and a pass needing more than
+ * 32768 floats per row (1048576 / 32) is rejected with YMC_ERR_ARG.  All calls on one device share
+ * that buffer: calls that overlap on one device race and must be serialised by the caller. */
+ymc_status ymc_mlp_forward_ensemble_noalloc(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows,
+                                            float *y);
 /* Timing hook: device milliseconds per fused ensemble pass of `rows` (<= 4096) rows already resident
  * on the device (zero inputs), averaged over `reps` launches timed with HIP events on the first
  * network's stream. */

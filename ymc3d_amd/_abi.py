@@ -162,6 +162,8 @@ _SIGS = {
     "ymc_mlp_get_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_uint64]),
     "ymc_mlp_forward": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float)]),
     "ymc_mlp_forward_ensemble": (C.c_int, [P(C.c_void_p), C.c_int32, P(C.c_float), C.c_int64, P(C.c_float)]),
+    "ymc_mlp_forward_ensemble_noalloc": (C.c_int, [P(C.c_void_p), C.c_int32, P(C.c_float), C.c_int64,
+                                                   P(C.c_float)]),
     "ymc_mlp_time_forward": (C.c_int, [P(C.c_void_p), C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]),
     "ymc_mlp_gradients": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float), P(C.c_float)]),
     "ymc_mlp_set_data": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64]),

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -507,21 +508,26 @@ bool fused_ok(const ymc_mlp *n) {
   return true;
 }
 
+// Allocates the transposed, padded weight copy of the fused pass (networks the fused pass serves;
+// done when the network is created, so the passes themselves allocate nothing).
+ymc_status alloc_wt(ymc_mlp *n) {
+  if (n->wt || !fused_ok(n)) return YMC_OK;
+  const int L = (int)n->widths.size() - 1;
+  size_t total = 0;
+  for (int l = 0; l < L; ++l) {
+    n->k16.push_back((n->widths[l] + 15) / 16 * 16);
+    n->npad.push_back((n->widths[l + 1] + 31) / 32 * 32);
+    n->wt_off.push_back(total);
+    total += (size_t)n->k16[l] * n->npad[l];
+  }
+  return dalloc(&n->wt, total);
+}
+
 // Rebuilds the transposed weight copy from the BF16 weights if they changed.
 ymc_status refresh_wt(ymc_mlp *n) {
   if (!n->wt_dirty) return YMC_OK;
   const int L = (int)n->widths.size() - 1;
-  if (!n->wt) {
-    size_t total = 0;
-    for (int l = 0; l < L; ++l) {
-      n->k16.push_back((n->widths[l] + 15) / 16 * 16);
-      n->npad.push_back((n->widths[l + 1] + 31) / 32 * 32);
-      n->wt_off.push_back(total);
-      total += (size_t)n->k16[l] * n->npad[l];
-    }
-    ymc_status st;
-    if ((st = dalloc(&n->wt, total))) return st;
-  }
+  if (!n->wt) return fail(YMC_ERR_STATE, "fused pass: packed weights missing");
   for (int l = 0; l < L; ++l) {
     const int cnt = n->k16[l] * n->npad[l];
     hipLaunchKernelGGL(transpose_pad, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, n->ph + n->w_off[l],
@@ -599,7 +605,7 @@ ymc_status ymc_mlp_create(int32_t device, const int32_t *widths, int32_t n_width
   ymc_status st;
   if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
       (st = dalloc(&n->p, n->n_params)) || (st = dalloc(&n->g, n->n_params)) || (st = dalloc(&n->m, n->n_params)) ||
-      (st = dalloc(&n->v, n->n_params)) || (st = dalloc(&n->ph, n->n_params))) {
+      (st = dalloc(&n->v, n->n_params)) || (st = dalloc(&n->ph, n->n_params)) || (st = alloc_wt(n))) {
     ymc_mlp_destroy(n);
     return fail(YMC_ERR_NOMEM, "ymc_mlp_create: allocation failed");
   }
@@ -662,14 +668,39 @@ ymc_status ymc_mlp_get_params(ymc_mlp *n, float *params, uint64_t count) {
 // Fused pass over `rows` host rows for `count` networks, in launches of kFusedChunk rows.
 constexpr int kFusedChunk = 4096;
 
-static ymc_status fused_forward(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y) {
+// Fused pass through a staging buffer `stage` of chunk * (in + count * out) floats on the networks'
+// device: each chunk of rows is copied in, run, and its outputs copied out.  Allocates nothing.
+static ymc_status fused_forward_staged(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y,
+                                       float *stage, int chunk) {
   ymc_mlp *n0 = nets[0];
-  MLP_TRY(hipSetDevice(n0->device));
   ymc_status st;
   for (int q = 0; q < count; ++q) {
     if ((st = refresh_wt(nets[q]))) return st;
     MLP_TRY(hipStreamSynchronize(nets[q]->stream));
   }
+  const int L = (int)n0->widths.size() - 1;
+  const int in = n0->widths[0], out = n0->widths[L];
+  float *yd = stage + (size_t)chunk * in;
+  for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+    const int b = (int)std::min<int64_t>(chunk, rows - r0);
+    MLP_TRY(hipMemcpyAsync(stage, x + r0 * in, (size_t)b * in * sizeof(float), hipMemcpyHostToDevice, n0->stream));
+    float *ys[kFusedNets];
+    for (int q = 0; q < count; ++q) ys[q] = yd + (size_t)q * chunk * out;
+    const FusedArgs a = fused_args(nets, count, stage, b, ys);
+    hipLaunchKernelGGL(mlp_forward_fused, dim3((b + kFusedRows - 1) / kFusedRows, count), dim3(256), 0, n0->stream, a);
+    for (int q = 0; q < count; ++q)
+      MLP_TRY(hipMemcpyAsync(y + ((size_t)q * rows + r0) * out, ys[q], (size_t)b * out * sizeof(float),
+                             hipMemcpyDeviceToHost, n0->stream));
+  }
+  MLP_TRY(hipGetLastError());
+  MLP_TRY(hipStreamSynchronize(n0->stream));
+  return YMC_OK;
+}
+
+static ymc_status fused_forward(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows, float *y) {
+  ymc_mlp *n0 = nets[0];
+  MLP_TRY(hipSetDevice(n0->device));
+  ymc_status st;
   const int L = (int)n0->widths.size() - 1;
   const int in = n0->widths[0], out = n0->widths[L];
   if (n0->xf_rows < kFusedChunk) {
@@ -679,20 +710,31 @@ static ymc_status fused_forward(ymc_mlp *const *nets, int32_t count, const float
     if ((st = dalloc(&n0->xf, (size_t)kFusedChunk * (in + kFusedNets * out)))) return st;
     n0->xf_rows = kFusedChunk;
   }
-  float *yd = n0->xf + (size_t)kFusedChunk * in;
-  for (int64_t r0 = 0; r0 < rows; r0 += kFusedChunk) {
-    const int b = (int)std::min<int64_t>(kFusedChunk, rows - r0);
-    MLP_TRY(hipMemcpyAsync(n0->xf, x + r0 * in, (size_t)b * in * sizeof(float), hipMemcpyHostToDevice, n0->stream));
-    float *ys[kFusedNets];
-    for (int q = 0; q < count; ++q) ys[q] = yd + (size_t)q * kFusedChunk * out;
-    const FusedArgs a = fused_args(nets, count, n0->xf, b, ys);
-    hipLaunchKernelGGL(mlp_forward_fused, dim3((b + kFusedRows - 1) / kFusedRows, count), dim3(256), 0, n0->stream, a);
-    for (int q = 0; q < count; ++q)
-      MLP_TRY(hipMemcpyAsync(y + ((size_t)q * rows + r0) * out, ys[q], (size_t)b * out * sizeof(float),
-                             hipMemcpyDeviceToHost, n0->stream));
+  return fused_forward_staged(nets, count, x, rows, y, n0->xf, kFusedChunk);
+}
+
+// Staging buffer of ymc_mlp_forward_ensemble_noalloc, one per device for the life of the process:
+// the inputs and outputs of one chunk of rows.  Host code never uses the name `scratch` as an
+// address; it uses the per-device address hipGetSymbolAddress returns (scratch_address).
+constexpr size_t kScratchFloats = size_t(1) << 20;  // 4 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static float *g_scratch_ptr[kScratchDevices];
+
+// Address of `scratch` on `device`, which must be the current device; resolved once per device.
+static ymc_status scratch_address(int device, float **out) {
+  if (device < 0 || device >= kScratchDevices) return fail(YMC_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p)
+      return fail(YMC_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(err));
+    g_scratch_ptr[device] = static_cast<float *>(p);
   }
-  MLP_TRY(hipGetLastError());
-  MLP_TRY(hipStreamSynchronize(n0->stream));
+  *out = g_scratch_ptr[device];
   return YMC_OK;
 }
 
@@ -727,6 +769,26 @@ ymc_status ymc_mlp_forward_ensemble(ymc_mlp *const *nets, int32_t count, const f
   if (st) return st;
   if (!x || !y || rows < 0) return fail(YMC_ERR_ARG, "ymc_mlp_forward_ensemble: bad arguments");
   return fused_forward(nets, count, x, rows, y);
+}
+
+ymc_status ymc_mlp_forward_ensemble_noalloc(ymc_mlp *const *nets, int32_t count, const float *x, int64_t rows,
+                                            float *y) {
+  ymc_status st = check_ensemble(nets, count);
+  if (st) return st;
+  if (!x || !y || rows < 0) return fail(YMC_ERR_ARG, "ymc_mlp_forward_ensemble_noalloc: bad arguments");
+  ymc_mlp *n0 = nets[0];
+  const int L = (int)n0->widths.size() - 1;
+  const size_t per_row = (size_t)n0->widths[0] + (size_t)count * n0->widths[L];
+  if (per_row * kFusedRows > kScratchFloats)
+    return fail(YMC_ERR_ARG,
+                "ymc_mlp_forward_ensemble_noalloc: inputs + count x outputs = %zu floats per row; the %zu-float "
+                "staging buffer takes at most %zu",
+                per_row, kScratchFloats, kScratchFloats / kFusedRows);
+  const int chunk = (int)std::min<size_t>(kFusedChunk, kScratchFloats / per_row);
+  MLP_TRY(hipSetDevice(n0->device));
+  float *stage = nullptr;
+  if ((st = scratch_address(n0->device, &stage))) return st;
+  return fused_forward_staged(nets, count, x, rows, y, stage, chunk);
 }
 
 ymc_status ymc_mlp_time_forward(ymc_mlp *const *nets, int32_t count, int32_t rows, int32_t reps, double *ms_per_pass) {
