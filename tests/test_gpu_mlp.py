@@ -90,3 +90,59 @@ def test_fused_ensemble_forward():
     finally:
         for n in nets:
             n.close()
+
+
+def test_fused_single_layer_rounds_operands_to_nearest_even():
+    """One linear layer (12 -> 33): the fused pass computes bf16(x) bf16(W) + b with FP32 accumulation, so
+    its outputs match float64 products of the RNE-rounded operands to FP32 accuracy; a truncating or
+    otherwise different conversion would be off by ~1e-3."""
+    rng = np.random.default_rng(8)
+    net = S.MLP((12, 33), seed=5)
+    try:
+        p = rng.normal(size=net.n_params).astype(np.float32)
+        net.set_params(p)
+        x = rng.uniform(-3, 3, (1000, 12)).astype(np.float32)
+        got = net.forward(x)
+        W, b = p[:12 * 33].reshape(12, 33), p[12 * 33:]
+        xb, Wb = _bf16(x).astype(np.float64), _bf16(W).astype(np.float64)
+        want = xb @ Wb + b
+        scale = np.abs(xb) @ np.abs(Wb) + np.abs(b)  # FP32 accumulation error is relative to this
+        assert np.max(np.abs(got - want) / scale) < 2e-6
+        # operands truncated instead of rounded would leave errors of order 1e-3 of the scale
+        xt = (x.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32).astype(np.float64)
+        assert np.max(np.abs(xt @ Wb + b - want) / scale) > 1e-4
+    finally:
+        net.close()
+
+
+def test_fused_gelu_equals_exact_gelu_in_bf16():
+    """The device GELU through the fused pass: layer 1 copies input 0 to 32 hidden units (weights 1,
+    bias 0), layer 2 copies hidden unit i to output i (weights 1), so output i = bf16(GELU(bf16(x0))).
+    Against the exact erf-based GELU rounded to BF16: within one BF16 unit or 1e-6 everywhere (the
+    device's erfc approximation has absolute error <= 1.5e-7, so far in the negative tail, where GELU
+    is below 1e-6, only the absolute error is small), and equal for at least 99.5% of the inputs
+    with |GELU| >= 1e-3."""
+    from scipy.special import erf
+
+    net = S.MLP((12, 32, 33), seed=1)
+    try:
+        W1 = np.zeros((12, 32), np.float32)
+        W1[0, :] = 1.0
+        W2 = np.zeros((32, 33), np.float32)
+        W2[np.arange(32), np.arange(32)] = 1.0
+        net.set_params(np.concatenate([W1.ravel(), np.zeros(32, np.float32), W2.ravel(), np.zeros(33, np.float32)]))
+        z = _bf16(np.linspace(-9, 9, 20001).astype(np.float32))
+        x = np.zeros((z.size, 12), np.float32)
+        x[:, 0] = z
+        got = net.forward(x)[:, :32]
+        assert np.all(got == got[:, :1])
+        zd = z.astype(np.float64)
+        want = _bf16((0.5 * zd * (1 + erf(zd / np.sqrt(2)))).astype(np.float32))
+        g = got[:, 0]
+        gu, wu = (v.view(np.int32).astype(np.int64) >> 16 for v in (g, want))  # BF16 bit patterns
+        close = ((np.abs(gu - wu) <= 1) & (np.sign(g) == np.sign(want))) | (np.abs(g - want) <= 1e-6)
+        assert np.all(close), z[~close][:10]
+        big = np.abs(want) >= 1e-3
+        assert np.mean(g[big] == want[big]) >= 0.995, np.mean(g[big] == want[big])
+    finally:
+        net.close()

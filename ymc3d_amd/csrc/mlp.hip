@@ -30,17 +30,33 @@ typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even: gfx950's v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU(z) = z Phi(z), Phi the standard normal CDF.  With q(z) = erfc(|z|/sqrt 2) / 2 (the normal
+// tail beyond |z|), z Phi(z) = max(z, 0) - q |z| for either sign of z.  erfc(x) for x >= 0 is
+// Abramowitz and Stegun 7.1.26 (Hastings): t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) e^{-x^2} with
+// t = 1 / (1 + p x), absolute error <= 1.5e-7.  Written in b = |z| sqrt(log2(e) / 2), so that
+// e^{-x^2} = 2^{-b^2} and p x = p' b with p' = p / sqrt(log2 e); the coefficients carry the 1/2.
+// The GELU so computed is within 2.2e-7 of the exact one (7e-8 |z|), far below BF16 resolution.
+struct NormalTail {
+  float q;  // erfc(|z| / sqrt 2) / 2
+  float e;  // e^{-z^2 / 2}
+};
+__device__ __forceinline__ NormalTail normal_tail(float z) {
+  const float b = fabsf(z) * 0.84932180028801907f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.27273748087922250f, b, 1.0f));
+  const float e = __builtin_amdgcn_exp2f(-b * b);
+  const float poly =
+      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.5307027145f, -0.7265760135f), 0.7107068705f), -0.142248368f), 0.127414796f);
+  return NormalTail{poly * e, e};
+}
+__device__ __forceinline__ float gelu(float z) { return fmaf(-normal_tail(z).q, fabsf(z), fmaxf(z, 0.0f)); }
+// d/dz z Phi(z) = Phi(z) + z phi(z), phi(z) = e^{-z^2/2} / sqrt(2 pi).
 __device__ __forceinline__ float gelu_grad(float z) {
-  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * __expf(-0.5f * z * z);
-  return cdf + z * pdf;
+  const NormalTail nt = normal_tail(z);
+  const float cdf = z >= 0.0f ? 1.0f - nt.q : nt.q;
+  return fmaf(z * 0.39894228040143268f, nt.e, cdf);
 }
 
 enum Epilogue { kStoreF32 = 0, kBiasGelu = 1, kBias = 2 };
@@ -222,22 +238,29 @@ __global__ void gather_rows(const float *X, const float *Y, const int *idx, int 
 // Fused ensemble forward pass: one workgroup carries kFusedRows rows of the input through every
 // layer of one network (blockIdx.y selects the network), the activations staying in LDS as BF16;
 // v_mfma_f32_32x32x16_bf16 products with FP32 accumulation read A (activations) from LDS and B
-// from a transposed, zero-padded BF16 weight copy wt_l [Npad_l][K16_l] (K rounded up to 16, N to
-// 32), so both operands are 16-byte loads.  Widths up to kFusedMax.
+// from a zero-padded BF16 weight copy in fragment order (K rounded up to kFusedK, N to 32): for
+// column tile c and k-step s, the 64 lanes' 16-byte B fragments lie one after another, so a wave's
+// fragment load reads 1 KB of consecutive memory.  Widths up to kFusedMax.
+//
+// Each of the kFusedWaves waves owns one 32-column tile of a layer (a layer of kFusedMax columns has
+// exactly kFusedWaves tiles; narrower layers leave waves idle).  A wave's B fragments form one
+// sequence over the layers it works in, read kFusedDepth k-steps ahead: the ring of fragments in
+// flight runs across layer ends, so the first fragments of the next layer load during this layer's
+// epilogue.  K is padded to a multiple of kFusedDepth k-steps, which keeps every ring slot a
+// compile-time register.  Two workgroups share a CU (4 waves per SIMD), so one workgroup's
+// epilogue (bias, GELU, BF16 conversion) overlaps the other's products.
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-// kFusedRowTiles row tiles of 32 per workgroup; the 4 waves split them and the column tiles.
-constexpr int kFusedRowTiles = 1;
-constexpr int kFusedRows = 32 * kFusedRowTiles, kFusedMax = 256, kFusedPitch = kFusedMax + 8, kFusedLayers = 8,
-              kFusedNets = 8;
-constexpr int kWavesPerRowTile = 4 / kFusedRowTiles;
-constexpr int kTilesPerWave = (kFusedMax / 32 + kWavesPerRowTile - 1) / kWavesPerRowTile;
+constexpr int kFusedRows = 32, kFusedMax = 256, kFusedPitch = kFusedMax + 8, kFusedLayers = 8, kFusedNets = 8;
+constexpr int kFusedWaves = kFusedMax / 32, kFusedThreads = 64 * kFusedWaves;
+constexpr int kFusedDepth = 4;             // k-steps of B fragments in flight per wave
+constexpr int kFusedK = 16 * kFusedDepth;  // K padding of the packed weights
 
 struct FusedLayer {
-  const uint16_t *wt;  // [npad][k16] BF16
+  const uint16_t *wt;  // [npad/32][k/16][64][8] BF16 fragments, k a multiple of kFusedK (pack_weights)
   const float *bias;   // [n]
-  int k16, n, npad;
+  int k, n, npad;
 };
 struct FusedNet {
   FusedLayer layer[kFusedLayers];
@@ -249,86 +272,121 @@ struct FusedArgs {
   int rows, in, L;
 };
 
-__global__ __launch_bounds__(256) void mlp_forward_fused(FusedArgs args) {
+// Barrier for LDS traffic only: the wave's LDS operations have completed, then the workgroup meets.
+// (__syncthreads also waits for every global load in flight, which would drain the prefetched B
+// fragments at each layer.)  The memory clobber keeps the compiler from moving LDS accesses across.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// The products of one layer for one 32x32 output tile: STEPS k-steps of 16, A (activations) from LDS
+// row `arow`, B from the ring of fragments in flight.  Each step refills its ring slot with the
+// fragment kFusedDepth steps ahead: this layer's (`cur`) or, in the last steps, the next layer's
+// (`nxt`).  Fully unrolled, so every slot and source is fixed at compile time.  A lane's fragments of
+// consecutive k-steps are kFragStep elements apart.
+constexpr int kFragStep = 64 * 8;
+template <int STEPS>
+__device__ __forceinline__ void layer_products(float16_t &acc, bf16x8_t (&ring)[kFusedDepth], const uint16_t *arow,
+                                               const uint16_t *cur, const uint16_t *nxt) {
+#pragma unroll
+  for (int s = 0; s < STEPS; ++s) {
+    const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(arow + 16 * s);
+    const bf16x8_t b = ring[s % kFusedDepth];
+    const uint16_t *src = s + kFusedDepth < STEPS ? cur + kFragStep * (s + kFusedDepth)
+                                                  : nxt + kFragStep * (s + kFusedDepth - STEPS);
+    ring[s % kFusedDepth] = *reinterpret_cast<const bf16x8_t *>(src);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+}
+
+__global__ __launch_bounds__(kFusedThreads) void mlp_forward_fused(FusedArgs args) {
   __shared__ __align__(16) uint16_t act[kFusedRows][kFusedPitch];
   const FusedNet &net = args.net[blockIdx.y];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int r0 = blockIdx.x * kFusedRows;
-  const int k16_0 = net.layer[0].k16;
-  for (int e = tid; e < kFusedRows * k16_0; e += 256) {
-    const int m = e / k16_0, k = e % k16_0;
+  const int lr = lane & 31, lk = (lane >> 5) * 8;
+  const int L = args.L;
+
+  // This wave's B fragments: those of column tile `wave` of wt_l, in the layers where the wave owns
+  // that tile.
+  auto owns = [&](int l) { return wave * 32 < net.layer[l].npad; };
+  auto base = [&](int l) {
+    return net.layer[l].wt + ((size_t)wave * (net.layer[l].k / 16) * 64 + lane) * 8;
+  };
+  auto next_owned = [&](int l) {
+    while (l < L && !owns(l)) ++l;
+    return l;
+  };
+  bf16x8_t ring[kFusedDepth];
+  int cl = next_owned(0);
+  if (cl < L) {
+    const uint16_t *p = base(cl);
+#pragma unroll
+    for (int d = 0; d < kFusedDepth; ++d) ring[d] = *reinterpret_cast<const bf16x8_t *>(p + kFragStep * d);
+  }
+
+  // inputs, BF16, zero outside the rows and the first `in` columns
+  const int k0w = net.layer[0].k;
+  for (int e = tid; e < kFusedRows * k0w; e += kFusedThreads) {
+    const int m = e / k0w, k = e % k0w;
     float v = 0.0f;
     if (r0 + m < args.rows && k < args.in) v = args.x[(size_t)(r0 + m) * args.in + k];
     act[m][k] = f2bf(v);
   }
-  __syncthreads();
-  // wave w takes row tile w / kWavesPerRowTile and its share of the 32-column tiles
-  const int rt = wave / kWavesPerRowTile, part = wave % kWavesPerRowTile;
-  const int lr = lane & 31, lk = (lane >> 5) * 8;
-  for (int l = 0; l < args.L; ++l) {
+  lds_barrier();
+
+  const uint16_t *arow = &act[lr][lk];
+  for (int l = 0; l < L; ++l) {
     const FusedLayer ly = net.layer[l];
-    const int ctiles = ly.npad / 32;
-    const int cpw = (ctiles + kWavesPerRowTile - 1) / kWavesPerRowTile;
-    const int c0 = part * cpw;
-    const int cn = min(cpw, ctiles - c0);  // may be <= 0 for narrow layers: this wave then idles
-    float16_t acc[kTilesPerWave];
-    for (int j = 0; j < kTilesPerWave; ++j)
-      for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
-    // The four B fragments of a step are loaded together (tiles past cn reload the last valid
-    // tile and their products are discarded), so their latencies overlap; the step loop is
-    // unrolled for the same reason.
-    if (cn > 0) {
-      const uint16_t *wrow[kTilesPerWave];
-#pragma unroll
-      for (int j = 0; j < kTilesPerWave; ++j)
-        wrow[j] = ly.wt + (size_t)((c0 + min(j, cn - 1)) * 32 + lr) * ly.k16 + lk;
-      bf16x8_t bn[kTilesPerWave];
-#pragma unroll
-      for (int j = 0; j < kTilesPerWave; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(wrow[j]);
-      for (int k0 = 0; k0 < ly.k16; k0 += 16) {
-        bf16x8_t b[kTilesPerWave];
-#pragma unroll
-        for (int j = 0; j < kTilesPerWave; ++j) b[j] = bn[j];
-        if (k0 + 16 < ly.k16) {  // next step's fragments in flight during this step's products
-#pragma unroll
-          for (int j = 0; j < kTilesPerWave; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(wrow[j] + k0 + 16);
-        }
-        const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&act[rt * 32 + lr][k0 + lk]);
-#pragma unroll
-        for (int j = 0; j < kTilesPerWave; ++j)
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+    const bool mine = l == cl;  // this wave owns column tile `wave` of this layer
+    const bool last = l + 1 == L;
+    const int col = wave * 32 + lr;
+    float16_t acc;
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    float bz = 0.0f;
+    if (mine) {
+      if (col < ly.n) bz = ly.bias[col];  // loaded ahead of the ring's refills
+      const int nl = next_owned(l + 1);
+      const uint16_t *cur = base(l);
+      const uint16_t *nxt = nl < L ? base(nl) : cur;  // past the last layer: harmless reloads
+      switch (ly.k) {
+        case 64: layer_products<4>(acc, ring, arow, cur, nxt); break;
+        case 128: layer_products<8>(acc, ring, arow, cur, nxt); break;
+        case 192: layer_products<12>(acc, ring, arow, cur, nxt); break;
+        default: layer_products<16>(acc, ring, arow, cur, nxt); break;
       }
+      cl = nl;
     }
-    __syncthreads();  // every wave has read this layer's input
-    const bool last = l + 1 == args.L;
-    const int next_k16 = last ? 0 : net.layer[l + 1].k16;
-#pragma unroll
-    for (int j = 0; j < kTilesPerWave; ++j) {
-      if (j >= cn) continue;
-      const int col = (c0 + j) * 32 + lr;
-      for (int v = 0; v < 16; ++v) {
-        const int row = rt * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (col < ly.n) {
-          const float z = acc[j][v] + ly.bias[col];
-          if (last) {
-            if (r0 + row < args.rows) net.y[(size_t)(r0 + row) * ly.n + col] = z;
-          } else {
-            act[row][col] = f2bf(gelu(z));
-          }
-        } else if (!last && col < next_k16) {
-          act[row][col] = 0;  // zero padding of the next layer's K
+    lds_barrier();  // every wave has read this layer's input
+    if (mine) {
+      if (last) {
+        for (int v = 0; v < 16; ++v) {
+          const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          if (col < ly.n && r0 + row < args.rows) net.y[(size_t)(r0 + row) * ly.n + col] = acc[v] + bz;
+        }
+      } else {
+        // columns n..npad-1 have zero weights and bias: GELU(0) = 0 pads the next layer's K
+        for (int v = 0; v < 16; ++v) {
+          const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          act[row][col] = f2bf(gelu(acc[v] + bz));
         }
       }
+    } else if (!last && col < net.layer[l + 1].k) {
+      for (int v = 0; v < 16; ++v) act[8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)][col] = 0;  // K padding
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
-// wt [npad][k16] from the row-major BF16 weights W [k][n], zero outside k < K, n < N.
-__global__ void transpose_pad(const uint16_t *W, int K, int N, uint16_t *wt, int k16, int npad) {
+// Fragment-order copy of the row-major BF16 weights W [K][N] for the fused pass (kpad a multiple of
+// 16, npad of 32): element j of lane l's B fragment for column tile c and k-step s,
+// wt[((c * kpad/16 + s) * 64 + l) * 8 + j], is W[16 s + 8 (l / 32) + j][32 c + l % 32], zero
+// outside k < K, n < N.
+__global__ void pack_weights(const uint16_t *W, int K, int N, uint16_t *wt, int kpad, int npad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npad * k16) return;
-  const int n = i / k16, k = i % k16;
+  if (i >= npad * kpad) return;
+  const int j = i & 7, l = (i >> 3) & 63, f = i >> 9;
+  const int s = f % (kpad / 16), c = f / (kpad / 16);
+  const int n = 32 * c + (l & 31), k = 16 * s + 8 * (l >> 5) + j;
   wt[i] = (n < N && k < K) ? W[(size_t)k * N + n] : (uint16_t)0;
 }
 
@@ -366,7 +424,7 @@ struct ymc_mlp {
   int64_t n_data = 0;
   int *idx = nullptr;
   int64_t adam_t = 0;
-  // transposed, padded BF16 weights for the fused forward pass; rebuilt when the weights change
+  // padded BF16 weights in fragment order for the fused forward pass; rebuilt when the weights change
   uint16_t *wt = nullptr;
   std::vector<size_t> wt_off;
   std::vector<int> k16, npad;
@@ -508,14 +566,14 @@ bool fused_ok(const ymc_mlp *n) {
   return true;
 }
 
-// Allocates the transposed, padded weight copy of the fused pass (networks the fused pass serves;
+// Allocates the fragment-order weight copy of the fused pass (networks the fused pass serves;
 // done when the network is created, so the passes themselves allocate nothing).
 ymc_status alloc_wt(ymc_mlp *n) {
   if (n->wt || !fused_ok(n)) return YMC_OK;
   const int L = (int)n->widths.size() - 1;
   size_t total = 0;
   for (int l = 0; l < L; ++l) {
-    n->k16.push_back((n->widths[l] + 15) / 16 * 16);
+    n->k16.push_back((n->widths[l] + kFusedK - 1) / kFusedK * kFusedK);
     n->npad.push_back((n->widths[l + 1] + 31) / 32 * 32);
     n->wt_off.push_back(total);
     total += (size_t)n->k16[l] * n->npad[l];
@@ -523,14 +581,14 @@ ymc_status alloc_wt(ymc_mlp *n) {
   return dalloc(&n->wt, total);
 }
 
-// Rebuilds the transposed weight copy from the BF16 weights if they changed.
+// Rebuilds the fragment-order weight copy from the BF16 weights if they changed.
 ymc_status refresh_wt(ymc_mlp *n) {
   if (!n->wt_dirty) return YMC_OK;
   const int L = (int)n->widths.size() - 1;
   if (!n->wt) return fail(YMC_ERR_STATE, "fused pass: packed weights missing");
   for (int l = 0; l < L; ++l) {
     const int cnt = n->k16[l] * n->npad[l];
-    hipLaunchKernelGGL(transpose_pad, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, n->ph + n->w_off[l],
+    hipLaunchKernelGGL(pack_weights, dim3((cnt + 255) / 256), dim3(256), 0, n->stream, n->ph + n->w_off[l],
                        n->widths[l], n->widths[l + 1], n->wt + n->wt_off[l], n->k16[l], n->npad[l]);
   }
   MLP_TRY(hipGetLastError());
@@ -552,7 +610,7 @@ FusedArgs fused_args(ymc_mlp *const *nets, int count, const float *x, int rows, 
     for (int l = 0; l < a.L; ++l) {
       a.net[q].layer[l].wt = n->wt + n->wt_off[l];
       a.net[q].layer[l].bias = n->p + n->b_off[l];
-      a.net[q].layer[l].k16 = n->k16[l];
+      a.net[q].layer[l].k = n->k16[l];
       a.net[q].layer[l].n = n->widths[l + 1];
       a.net[q].layer[l].npad = n->npad[l];
     }
@@ -687,7 +745,8 @@ static ymc_status fused_forward_staged(ymc_mlp *const *nets, int32_t count, cons
     float *ys[kFusedNets];
     for (int q = 0; q < count; ++q) ys[q] = yd + (size_t)q * chunk * out;
     const FusedArgs a = fused_args(nets, count, stage, b, ys);
-    hipLaunchKernelGGL(mlp_forward_fused, dim3((b + kFusedRows - 1) / kFusedRows, count), dim3(256), 0, n0->stream, a);
+    hipLaunchKernelGGL(mlp_forward_fused, dim3((b + kFusedRows - 1) / kFusedRows, count), dim3(kFusedThreads), 0,
+                       n0->stream, a);
     for (int q = 0; q < count; ++q)
       MLP_TRY(hipMemcpyAsync(y + ((size_t)q * rows + r0) * out, ys[q], (size_t)b * out * sizeof(float),
                              hipMemcpyDeviceToHost, n0->stream));
@@ -809,7 +868,7 @@ ymc_status ymc_mlp_time_forward(ymc_mlp *const *nets, int32_t count, int32_t row
   MLP_TRY(hipEventCreate(&e0));
   MLP_TRY(hipEventCreate(&e1));
   MLP_TRY(hipEventRecord(e0, n0->stream));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(mlp_forward_fused, grid, dim3(256), 0, n0->stream, a);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(mlp_forward_fused, grid, dim3(kFusedThreads), 0, n0->stream, a);
   MLP_TRY(hipEventRecord(e1, n0->stream));
   MLP_TRY(hipGetLastError());
   MLP_TRY(hipEventSynchronize(e1));
