@@ -55,6 +55,9 @@ def main() -> int:
     ap.add_argument("--chain-lanes", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="default workload only: skip the secondary workloads (slab with inclusion, batched "
+                         "spectra) and the surrogate serving measurement")
     ap.add_argument("--absorption", action="store_true", help="also tally the per-voxel absorption grid")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the workload, time a plain device-to-device copy of this many bytes (the "
@@ -73,48 +76,94 @@ def main() -> int:
               "the fixed-point tallies, which this engine does not run; use --gpus 1", file=sys.stderr)
         return 2
 
+    if args.workload == "surrogate":
+        return surrogate_bench(args)
+    r = transport(args.workload, args.photons, args, args.steps, args.warmup, absorption=args.absorption,
+                  pool_lanes=args.pool_lanes, chain_lanes=args.chain_lanes)
+    out = {
+        "metric": "photon packets/s (1 GPU)",
+        "value": r["value"],
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": r["config"],
+        "roofline": r["roofline"],
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(r["scene"], r["tab"], r["params"], r["photons"], args.cpu_seconds)
+    default_run = args.workload == "skin_vessels" and not (args.photons or args.pool_lanes or args.chain_lanes or
+                                                           args.absorption)
+    if default_run and not args.no_secondary:
+        # the other transport workloads, each with its CPU baseline on a bounded sample
+        out["secondary"] = []
+        for name in ("skin_slab_sphere", "batched_spectra"):
+            s = transport(name, 0, args, args.steps, args.warmup)
+            entry = {"workload": s["config"]["workload"], "value": s["value"], "unit": "photons/s",
+                     "ms_per_step": s["ms_per_step"], "kernel_ms": s["config"]["kernel_ms"],
+                     "photons_per_step": s["photons"], "tables": s["config"]["tables"],
+                     "steps_per_photon": s["config"]["steps_per_photon"], "roofline_frac": s["roofline"]["frac"],
+                     "spectra_per_s": s["config"]["spectra_per_s"], "launch_params": s["config"]["launch_params"],
+                     "cpu_baseline": None}
+            if not args.no_cpu:
+                cb = cpu_baseline(s["scene"], s["tab"], s["params"], s["photons"], args.cpu_seconds / 2)
+                entry["cpu_baseline"] = cb
+                entry["vs_cpu"] = s["value"] / cb["value"]
+            out["secondary"].append(entry)
+        out["surrogate_serving"] = surrogate_serving()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def transport(workload, photons, args, steps, warmup, absorption=False, pool_lanes=0, chain_lanes=0) -> dict:
+    """One transport workload: a counter pass (steps and Fresnel events for the roofline numerator),
+    `warmup` untimed and `steps` timed runs of the whole budget (tally reset, persistent kernel,
+    tally read-back), inputs resident on the device."""
     import numpy as np
 
     import ymc3d_amd as y
-    from ymc3d_amd import configs
+    from ymc3d_amd import autotune, configs
 
-    if args.workload == "surrogate":
-        return surrogate_bench(args)
-    fn, default_n = WORKLOADS[args.workload]
+    fn, default_n = WORKLOADS[workload]
     if fn == "batched_spectra":
         scene = configs.batched_spectra()
         n_photons = int(np.sum(scene.params["budgets"]))
     else:
-        n_photons = args.photons or default_n
+        n_photons = photons or default_n
         scene = getattr(configs, fn)(photons=n_photons)
     params = dict(scene.params)
-    from ymc3d_amd import autotune
-
     params.update(autotune.tuned(scene, scene.params))
-    if args.pool_lanes:
-        params["pool_lanes"] = args.pool_lanes
-    if args.chain_lanes:
-        params["chain_lanes"] = args.chain_lanes
+    if pool_lanes:
+        params["pool_lanes"] = pool_lanes
+    if chain_lanes:
+        params["chain_lanes"] = chain_lanes
     tab = y.default_multipliers()
     eng = y.Engine(0, tab)
     scene.load(eng)
 
     # Counter pass (deterministic: the same chains and streams as every timed step).
-    extra = y.FLAG_ABSORPTION if args.absorption else 0
+    extra = y.FLAG_ABSORPTION if absorption else 0
     eng.prepare(**params, flags=y.FLAG_COUNTERS | extra)
     eng.run()
     counted = eng.read_tally()
-    steps = int(counted.totals["steps"].sum())
+    n_steps = int(counted.totals["steps"].sum())
     fresnel = int(counted.totals["fresnel"].sum())
     tables = len(counted.totals)
 
     eng.prepare(**params, flags=extra)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         eng.run()
         eng.read_tally()
     kernel_ms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         eng.run()
         res = eng.read_tally()
         kernel_ms.append(res.kernel_ms)
@@ -123,22 +172,17 @@ def main() -> int:
     # every step runs the same chains and streams: its maps equal the counter pass's bit for bit
     assert np.array_equal(res.R_w, counted.R_w) and np.array_equal(res.T_w, counted.T_w)
 
-    value = args.steps * n_photons / elapsed
     kms = float(np.mean(kernel_ms))
-    achieved = (steps * FLOPS_PER_STEP + fresnel * FLOPS_PER_FRESNEL) / (kms * 1e-3) / 1e12
-    out = {
-        "metric": "photon packets/s (1 GPU)",
-        "value": value,
-        "unit": "photons/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
+    achieved = (n_steps * FLOPS_PER_STEP + fresnel * FLOPS_PER_FRESNEL) / (kms * 1e-3) / 1e12
+    pool = eng.pool_lanes()
+    eng.close()
+    return {
+        "value": steps * n_photons / elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "photons": n_photons,
+        "scene": scene,
+        "params": params,
+        "tab": tab,
         "config": {
             "workload": scene.name,
             "photons_per_step": n_photons,
@@ -146,13 +190,13 @@ def main() -> int:
             "voxels": list(scene.shape),
             "voxel_cm": scene.voxel[0],
             "tally": [params.get("tally_nx", 0) or scene.shape[0], params.get("tally_ny", 0) or scene.shape[1]],
-            "pool_lanes": eng.pool_lanes(),
+            "pool_lanes": pool,
             "tables": tables,
-            "spectra_per_s": args.steps / elapsed if tables > 1 else None,
-            "absorption_grid": bool(args.absorption),
+            "spectra_per_s": steps / elapsed if tables > 1 else None,
+            "absorption_grid": bool(absorption),
             "launch_params": {k: params[k] for k in ("pool_lanes", "jump_max_crossings", "relaunch_lanes",
                                                       "chain_lanes") if k in params},
-            "steps_per_photon": steps / n_photons,
+            "steps_per_photon": n_steps / n_photons,
             "fresnel_per_photon": fresnel / n_photons,
             "kernel_ms": kms,
         },
@@ -162,16 +206,45 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None if args.absorption else MEASURED_TRAFFIC.get((args.workload, n_photons, eng.pool_lanes())),
+            "traffic": None if absorption else MEASURED_TRAFFIC.get((workload, n_photons, pool)),
         },
-        "cpu_baseline": None,
     }
-    eng.close()
 
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, tab, params, n_photons, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
-    return 0
+
+def surrogate_serving() -> dict:
+    """Surrogate serving of the 256 x 256-pixel x 64-wavelength hyperspectral cube with the shipped
+    ensemble: every query through the ensemble (fused passes of 4096), 1e6-packet Monte Carlo
+    fallback for flagged queries and a 1% audit; the Monte Carlo runs on seeded samples of 256 flagged
+    and 256 audit queries and its time for the whole cube is scaled from them
+    (surrogate_pipeline.measure_cube).  Plain Monte Carlo at matched accuracy: the audit sample's
+    packet rate over 64 x n_star packets per spectrum."""
+    import ymc3d_amd as y
+    from ymc3d_amd import surrogate as S, surrogate_pipeline as SP
+
+    if not os.path.exists(S.DEFAULT_ENSEMBLE):
+        return {"error": f"ensemble file {S.DEFAULT_ENSEMBLE} missing"}
+    ens, extra = S.Ensemble.load(S.DEFAULT_ENSEMBLE)
+    eng = y.Engine(0)
+    try:
+        r = SP.measure_cube(ens, eng, float(extra["threshold"]), float(extra["n_star"]), cube=(256, 256),
+                            fallback_sample=256, audit_sample=256, seed=0)
+    finally:
+        ens.close()
+        eng.close()
+    keep = ("cube", "queries", "outlier_pixels_frac", "flagged_frac", "out_of_range", "audit", "fallback_photons",
+            "fallback_sample", "audit_sample", "surrogate_s", "surrogate_queries_per_s", "fallback_s_per_query",
+            "mc_s_whole_cube", "total_s", "spectra_per_s_with_fallback", "surrogate_only_spectra_per_s",
+            "audit_median_rel_err", "audit_mc_median_rel_se", "audit_rel_err_less_mc_noise", "mc_packets_per_s",
+            "mc_spectra_per_s_at_matched_error", "speedup_with_fallback")
+    out = {"workload": "surrogate_serving_256x256x64", "unit": "spectra/s", "ensemble": os.path.basename(S.DEFAULT_ENSEMBLE)}
+    out.update({k: (r[k] if not hasattr(r[k], "tolist") else r[k].tolist()) for k in keep if k in r})
+    out["value"] = out.get("spectra_per_s_with_fallback")
+    for k in ("median_rel_err", "median_rel_err_total", "test_photons", "n_star"):
+        if k in extra:
+            out["eval_" + k] = float(extra[k])
+    out["note"] = ("Monte Carlo time of the whole cube scaled from the per-query time of the seeded fallback and "
+                   "audit samples; eval_* from the held-out evaluation stored with the ensemble")
+    return out
 
 
 def surrogate_bench(args) -> int:
@@ -240,7 +313,7 @@ def surrogate_bench(args) -> int:
         out["cpu_baseline"] = {"value": done / members / dt / 64, "unit": "spectra/s",
                                "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
                                "kind": "port", "sample": f"{done} rows through one float64 numpy network, {dt:.1f} s, "
-                                                         f"scaled to {members} members"}
+                                                         f"scaled to {members} members", **host_cpu()}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -316,7 +389,23 @@ def cpu_baseline(scene, tab, params, n_photons, seconds):
     sample = run(sample)
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"{sample} packets of {scene.name} (same geometry, tables, schedule rule), {dt:.1f} s"}
+            "sample": f"{sample} packets of {scene.name} (same geometry, tables, schedule rule), {dt:.1f} s",
+            **host_cpu()}
+
+
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: /proc/cpuinfo model name, logical CPUs online (nproc) and the
+    CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@ fall back to Monte Carlo.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -18,6 +19,12 @@ from .engine import check, load_library
 
 WIDTHS = (12, 256, 256, 256, 33)
 LOG_FLOOR = 1e-7  # reflectance per packet below this is treated as this when taking log10
+
+# Trained ensemble shipped with the package (Ensemble.save format, with `threshold`, `n_star` and
+# `mc_packets_per_s`): 4 members trained on 2e5 samples x 1e5 packets of the two-layer-plus-vessel
+# model by `python -m ymc3d_amd.surrogate_pipeline --samples 200000 --photons 100000 --members 4
+# --epochs 40 --test 512 --test-photons 4000000 --cube 0 0 --save-ensemble <path>`.
+DEFAULT_ENSEMBLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_skin2_ensemble.npz")
 
 
 class MLP:
@@ -182,6 +189,24 @@ class Ensemble:
         return ens, data
 
 
+def triage(ens: Ensemble, P: np.ndarray, threshold: float, audit_frac: float = 0.0, seed: int = 0) -> dict:
+    """The surrogate half of `serve`: ensemble mean and spread of every row of P (fused passes of 4096
+    rows), the rows to re-run by Monte Carlo (`flagged`: outside the training ranges, or largest member
+    standard deviation over the 33 outputs above `threshold`, log10 units) and a seeded `audit_frac`
+    share of the others (`audit`).  `surrogate_s` is the wall time of the inference."""
+    import time
+
+    P = np.atleast_2d(P)
+    t0 = time.perf_counter()
+    mean, spread = ens.predict(P)
+    t_sur = time.perf_counter() - t0
+    flagged = (~in_range(P)) | (spread.max(1) > threshold)
+    audit = np.zeros(len(P), bool)
+    if audit_frac > 0:
+        audit = (~flagged) & (np.random.default_rng(seed).uniform(size=len(P)) < audit_frac)
+    return {"mean": mean, "spread": spread, "flagged": flagged, "audit": audit, "surrogate_s": t_sur}
+
+
 def serve(ens: Ensemble, P: np.ndarray, engine, threshold: float, fallback_photons: int, audit_frac: float = 0.0,
           seed: int = 0, **mc_kw) -> dict:
     """Answers parameter rows P with the surrogate, falling back to Monte Carlo.
@@ -192,18 +217,11 @@ def serve(ens: Ensemble, P: np.ndarray, engine, threshold: float, fallback_photo
     re-run too, to measure the served error; those keep the surrogate answer.  Returns `pred`
     (log10 targets, Monte Carlo values in flagged rows), `flagged`, `audit`, `rows` (indices
     re-run), `mc` (the run_mc result for `rows`) and `surrogate_s` (inference seconds)."""
-    import time
-
     from . import surrogate_data
 
     P = np.atleast_2d(P)
-    t0 = time.perf_counter()
-    mean, spread = ens.predict(P)
-    t_sur = time.perf_counter() - t0
-    flagged = (~in_range(P)) | (spread.max(1) > threshold)
-    audit = np.zeros(len(P), bool)
-    if audit_frac > 0:
-        audit = (~flagged) & (np.random.default_rng(seed).uniform(size=len(P)) < audit_frac)
+    tr = triage(ens, P, threshold, audit_frac, seed)
+    mean, spread, flagged, audit, t_sur = tr["mean"], tr["spread"], tr["flagged"], tr["audit"], tr["surrogate_s"]
     rows = np.flatnonzero(flagged | audit)
     mc = None
     pred = mean.copy()

@@ -106,8 +106,69 @@ def main(argv=None) -> int:
     rep["evaluation"]["spread_threshold_log10"] = threshold
     print("evaluation", json.dumps(rep["evaluation"]), flush=True)
     if a.save_ensemble:
-        ens.save(a.save_ensemble, threshold=threshold, n_star=n_star, mc_packets_per_s=rep["data"]["packets_per_s"])
+        ens.save(a.save_ensemble, threshold=threshold, n_star=n_star, mc_packets_per_s=rep["data"]["packets_per_s"],
+                 median_rel_err=med, median_rel_err_total=rep["evaluation"]["median_rel_err_total"],
+                 test_photons=a.test_photons, test_samples=a.test,
+                 mc_reference_median_rel_se=rep["evaluation"]["mc_reference_median_rel_se"])
     return serve_and_compare(a, rep, eng, ens, threshold, n_star)
+
+
+def measure_cube(ens, eng, threshold: float, n_star: float, cube=(256, 256), wavelengths: int = 64,
+                 outlier_frac: float = 0.05, fallback_photons: int = 1_000_000, audit_frac: float = 0.01,
+                 fallback_sample: int = 256, audit_sample: int = 256, seed: int = 0) -> dict:
+    """Serving a hyperspectral cube with Monte Carlo fallback, measured on the whole cube.
+
+    Every query of the height x width x `wavelengths` cube (`outlier_frac` of the pixels out of range)
+    goes through the ensemble (fused passes of 4096 rows) and is triaged (S.triage).  The
+    `fallback_photons` Monte Carlo runs on a seeded sample of `fallback_sample` flagged queries and of
+    `audit_sample` audit queries; the per-query fallback and audit times of those samples, times the
+    numbers of flagged and audit queries, give the Monte Carlo time of the whole cube.  The audit
+    sample gives the surrogate's median relative error against Monte Carlo (with that Monte Carlo's
+    own noise, and the error left once that noise is removed in quadrature), and its packet rate
+    gives plain Monte Carlo's spectra/s at the surrogate's accuracy (`n_star` packets per query: the
+    count at which plain Monte Carlo's median relative error equals the surrogate's)."""
+    from ymc3d_amd import surrogate as S, surrogate_data as D
+
+    H, W = cube
+    Q, outl = D.hyperspectral_cube(H, W, wavelengths, outlier_frac, seed=seed + 200)
+    tr = S.triage(ens, Q, threshold, audit_frac, seed=seed + 300)
+    flagged, audit = tr["flagged"], tr["audit"]
+    rng = np.random.default_rng(seed + 400)
+
+    def sample(mask, k):
+        idx = np.flatnonzero(mask)
+        return np.sort(rng.choice(idx, size=min(k, idx.size), replace=False)) if idx.size else idx
+
+    fs, aus = sample(flagged, fallback_sample), sample(audit, audit_sample)
+    mc_kw = dict(tables_per_launch=1024, progress_every=1e9)
+    # untimed warm-up launch (first use of the parametric kernel and of the fallback engines)
+    D.run_mc(eng, Q[fs[:8]] if fs.size else Q[:8], 10_000, seed=seed + 700, label="warm-up", **mc_kw)
+    fb = D.run_mc(eng, Q[fs], fallback_photons, seed=seed + 500, label="fallback-sample", **mc_kw) if fs.size else None
+    au = D.run_mc(eng, Q[aus], fallback_photons, seed=seed + 600, label="audit-sample", **mc_kw) if aus.size else None
+    out = {"cube": [H, W, wavelengths], "queries": int(len(Q)), "outlier_pixels_frac": float(outl.mean()),
+           "surrogate_s": tr["surrogate_s"], "surrogate_queries_per_s": len(Q) / tr["surrogate_s"],
+           "flagged": int(flagged.sum()), "flagged_frac": float(flagged.mean()),
+           "out_of_range": int((~S.in_range(Q)).sum()), "audit": int(audit.sum()),
+           "fallback_photons": fallback_photons, "fallback_sample": int(fs.size), "audit_sample": int(aus.size),
+           "mean": tr["mean"], "spread": tr["spread"], "flagged_mask": flagged, "audit_mask": audit,
+           "outlier_mask": outl, "Q": Q, "fallback_rows": fs, "audit_rows": aus, "fallback_mc": fb, "audit_mc": au}
+    t_fb = fb["wall_s"] / fs.size * flagged.sum() if fb else 0.0
+    t_au = au["wall_s"] / aus.size * audit.sum() if au else 0.0
+    out["fallback_s_per_query"] = fb["wall_s"] / fs.size if fb else 0.0
+    out["mc_s_whole_cube"] = t_fb + t_au
+    out["total_s"] = tr["surrogate_s"] + t_fb + t_au
+    out["spectra_per_s_with_fallback"] = H * W / out["total_s"]
+    out["surrogate_only_spectra_per_s"] = H * W / tr["surrogate_s"]
+    if au:
+        err, ref = rel_err(tr["mean"][aus], au["radial"], au["total"])
+        ase = np.concatenate([au["radial_se"], au["total_se"][:, None]], axis=1) / np.maximum(ref, 1e-12)
+        out["audit_median_rel_err"] = float(np.median(err))
+        out["audit_mc_median_rel_se"] = float(np.median(ase))
+        out["audit_rel_err_less_mc_noise"] = float(np.sqrt(max(np.median(err) ** 2 - np.median(ase) ** 2, 0.0)))
+        out["mc_packets_per_s"] = au["packets"] / au["wall_s"]
+        out["mc_spectra_per_s_at_matched_error"] = out["mc_packets_per_s"] / (wavelengths * n_star)
+        out["speedup_with_fallback"] = out["spectra_per_s_with_fallback"] / out["mc_spectra_per_s_at_matched_error"]
+    return out
 
 
 def serve_and_compare(a, rep, eng, ens, threshold, n_star) -> int:
