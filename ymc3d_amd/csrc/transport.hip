@@ -112,13 +112,15 @@ __device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t 
   return u;
 }
 
-// Per-lane skip table: entry 3(L-1) + d - 1 (L = 1, 2; d = 1..3) holds E(d*4^L), where E(n) =
+// Per-lane skip table: entry k - 1 (k = 1..7) holds E(4k) and entry 7 holds E(32), where E(n) =
 // 2^-128n * 2^64 mod p skips the draws of n crossings (4n steps) in one mwc_mulmod.  E(4) is the
 // state 14 MWC steps after v = 1 (2^-32*14 = 2^-128*4 * 2^64); the others follow from
-// mwc_mulmod(E(a), E(b)) = E(a + b).  tab points at the lane's first entry; entries are STRIDE
-// apart (one column per lane of the workgroup).
-constexpr int kSkipEntries = 6;
-constexpr int kSkipMax = 63;  // crossings one skip covers (three base-4 digits)
+// mwc_mulmod(E(a), E(b)) = E(a + b).  A skip of n = d + 4k + 32m crossings (d, k, m its digits in
+// bases 4, 8, 2) steps d crossings and takes one product for k and one for m: jumps of fewer than 32
+// crossings, nearly all of them, need one product.  tab points at the lane's first entry; entries are
+// STRIDE apart (one column per lane of the workgroup).
+constexpr int kSkipEntries = 8;
+constexpr int kSkipMax = 63;  // crossings one skip covers
 template <int STRIDE>
 __device__ __forceinline__ void build_skip(uint64_t *tab, uint32_t a) {
   uint64_t e4 = 1;
@@ -126,27 +128,31 @@ __device__ __forceinline__ void build_skip(uint64_t *tab, uint32_t a) {
   const uint64_t e8 = mwc_mulmod(e4, e4, a);
   const uint64_t e12 = mwc_mulmod(e8, e4, a);
   const uint64_t e16 = mwc_mulmod(e8, e8, a);
+  const uint64_t e20 = mwc_mulmod(e16, e4, a);
+  const uint64_t e24 = mwc_mulmod(e12, e12, a);
+  const uint64_t e28 = mwc_mulmod(e24, e4, a);
   const uint64_t e32 = mwc_mulmod(e16, e16, a);
-  const uint64_t e48 = mwc_mulmod(e32, e16, a);
   tab[0] = e4;
   tab[STRIDE] = e8;
   tab[2 * STRIDE] = e12;
   tab[3 * STRIDE] = e16;
-  tab[4 * STRIDE] = e32;
-  tab[5 * STRIDE] = e48;
+  tab[4 * STRIDE] = e20;
+  tab[5 * STRIDE] = e24;
+  tab[6 * STRIDE] = e28;
+  tab[7 * STRIDE] = e32;
 }
 // Advances the stream by the draws of n plain crossings, 0 <= n <= kSkipMax.
 template <int STRIDE>
 __device__ __forceinline__ void skip_crossings(uint64_t &rx, uint32_t ra, const uint64_t *tab, int n) {
-  const int d0 = n & 3, d1 = (n >> 2) & 3, d2 = (n >> 4) & 3;
-  for (int q = 0; q < d0; ++q) {
+  const int d = n & 3, k = (n >> 2) & 7, m = n >> 5;
+  for (int q = 0; q < d; ++q) {
     mwc_next(rx, ra);
     mwc_next(rx, ra);
     mwc_next(rx, ra);
     mwc_next(rx, ra);
   }
-  if (d1) rx = mwc_mulmod(rx, tab[(d1 - 1) * STRIDE], ra);
-  if (d2) rx = mwc_mulmod(rx, tab[(d2 + 2) * STRIDE], ra);
+  if (k) rx = mwc_mulmod(rx, tab[(k - 1) * STRIDE], ra);
+  if (m) rx = mwc_mulmod(rx, tab[7 * STRIDE], ra);
 }
 
 __device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
@@ -187,6 +193,13 @@ constexpr int kBlock = 256;  // threads per workgroup of the transport kernel
 struct RowConst {
   float4 r0, r1, r2;
 };
+
+// Row idx of a RowConst table, addressed with a 24-bit multiply (full rate, where a 32-bit integer
+// multiply issues at a quarter of it); row indices stay below 2^24 (checked when the tables are set).
+__device__ __forceinline__ const RowConst &row_at(const RowConst *rows, uint32_t idx) {
+  return *reinterpret_cast<const RowConst *>(reinterpret_cast<const char *>(rows) +
+                                             __umul24(idx, (uint32_t)sizeof(RowConst)));
+}
 
 struct KArgs {
   // geometry: one word per voxel holding its row and safe box (extents zup, zdn, rxy).  Grids with
@@ -387,17 +400,18 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
 }
 
 // Voxel of coordinate c along one axis, clamped to [lo, hi].  Rounding can put it one voxel off
-// next to a face; the jump below checks the position it commits against the voxel it names.
+// next to a face; the jump below checks the position it commits against the voxel it names, so any
+// voxel of the box is safe here.  The clamp is in integers (the conversion saturates; NaN gives 0).
 __device__ __forceinline__ int axis_voxel(float c, float inv_d, int lo, int hi) {
-  const float q = floorf(c * inv_d);
-  return (int)fminf(fmaxf(q, (float)lo), (float)hi);
+  const int q = (int)floorf(c * inv_d);
+  return min(max(q, lo), hi);
 }
 
-// Coordinate c with index e is a state the face test continues exactly: inside the voxel's
-// closed extent and strictly before the face ahead (so the next step has a positive length).
-__device__ __forceinline__ bool axis_ok(float c, int e, float d, bool up) {
-  const float lo = (float)e * d, hi = (float)(e + 1) * d;
-  return up ? ((c >= lo) & (c < hi)) : ((c > lo) & (c <= hi));
+// Coordinate c in the voxel with faces lo = e*d and hi = (e+1)*d is a state the face test continues
+// exactly: inside the voxel's closed extent and strictly before the face ahead (so the next step has
+// a positive length).  Compares only, no branches.
+__device__ __forceinline__ bool axis_ok(float c, float lo, float hi, bool up) {
+  return (c >= lo) & (c <= hi) & (c != (up ? hi : lo));
 }
 
 // Flight jump.  Inside the safe box of the anchor voxel every voxel has the anchor's tissue, no
@@ -432,10 +446,13 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   const int ei = axis_voxel(fmaf(p.ux, t_stop, p.x), a.idx, loi, hii);
   const int ej = axis_voxel(fmaf(p.uy, t_stop, p.y), a.idy, loj, hij);
   const int ek = axis_voxel(fmaf(p.uz, t_stop, p.z), a.idz, lok, hik);
-  // last face crossed on each axis: the face entering voxel e
-  const float fx = (float)(upx ? ei : ei + 1) * a.dx;
-  const float fy = (float)(upy ? ej : ej + 1) * a.dy;
-  const float fz = (float)(upz ? ek : ek + 1) * a.dz;
+  // faces of voxel e on each axis; the last face crossed is the one entering it
+  const float lox = (float)ei * a.dx, hix = (float)(ei + 1) * a.dx;
+  const float loy = (float)ej * a.dy, hiy = (float)(ej + 1) * a.dy;
+  const float loz = (float)ek * a.dz, hiz = (float)(ek + 1) * a.dz;
+  const float fx = upx ? lox : hix;
+  const float fy = upy ? loy : hiy;
+  const float fz = upz ? loz : hiz;
   const float tlx = (ei != p.i) ? (fx - p.x) * p.iux : 0.0f;
   const float tly = (ej != p.j) ? (fy - p.y) * p.iuy : 0.0f;
   const float tlz = (ek != p.k) ? (fz - p.z) * p.iuz : 0.0f;
@@ -453,8 +470,8 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   // one predicate without short-circuit branches
   const int cap = skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
   const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= cap) &
-                  !on_face & in_box(p, p.i, p.j, p.k) & axis_ok(qx, ei, a.dx, upx) &
-                  axis_ok(qy, ej, a.dy, upy) & axis_ok(qz, ek, a.dz, upz) & (qx > 0.0f) & (qx < a.Lx) &
+                  !on_face & in_box(p, p.i, p.j, p.k) & axis_ok(qx, lox, hix, upx) &
+                  axis_ok(qy, loy, hiy, upy) & axis_ok(qz, loz, hiz, upz) & (qx > 0.0f) & (qx < a.Lx) &
                   (qy > 0.0f) & (qy < a.Ly) & (dbm <= p.s) & (p.nstep + (uint32_t)n < kMaxSteps);
   if (ok) {
     p.x = qx;
@@ -488,10 +505,11 @@ enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
 // One step of a live packet.
 template <bool COUNTERS, bool LAYERED>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
-                                            const RowConst *tab, unsigned long long *tot, LaneAcc &acc,
-                                            const ymc_layered *lg, uint32_t table, float4 nrow) {
-  const float4 c0 = tab[p.t].r0;
-  const float4 c1 = tab[p.t].r1;
+                                            const RowConst *rows, uint32_t rbase, unsigned long long *tot,
+                                            LaneAcc &acc, const ymc_layered *lg, uint32_t table, float4 nrow) {
+  // this table's rows start at rows[rbase]
+  const float4 c0 = row_at(rows, rbase + p.t).r0;
+  const float4 c1 = row_at(rows, rbase + p.t).r1;
   const float ni = c0.w;
   const float mu_t = c1.x;
 
@@ -536,7 +554,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   if (LAYERED && a.num_types <= 4)
     nt = next_t == 0 ? nrow.x : (next_t == 1 ? nrow.y : (next_t == 2 ? nrow.z : nrow.w));
   else
-    nt = tab[next_t].r0.w;
+    nt = row_at(rows, rbase + next_t).r0.w;
 
   float nux = p.ux, nuy = p.uy, nuz = p.uz;
   bool refl = false;
@@ -593,7 +611,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
         if (COUNTERS) atomicAdd(&tot[kRoul], (unsigned long long)((long long)fx(Wr) - (long long)fx(W)));
         W = Wr;
       }
-      const float4 c2 = tab[p.t].r2;
+      const float4 c2 = row_at(rows, rbase + p.t).r2;
       // HG (:282-294)
       const float xi = u32_oc(u_cos);
       float ct = c2.x * rcp(fmaf(c2.z, xi, c2.y));
@@ -675,8 +693,9 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   bool zero_kill = false;
   if (db == 0.0f)  // rare: the launch-boundary test runs only for lanes that need it
     zero_kill = a.kill0 == 2 || (a.kill0 == 1 && (p.nstep % kLaunchSteps) == 0);
-  const bool bad = !isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z) || !isfinite(p.s) || !isfinite(p.ux) ||
-                   !isfinite(p.uy) || !isfinite(p.uz) || zero_kill || db < 0.0f || p.nstep >= kMaxSteps;
+  // bitwise, not short-circuit: every test is a compare, and no branch splits the reset below
+  const bool bad = !isfinite(p.x) | !isfinite(p.y) | !isfinite(p.z) | !isfinite(p.s) | !isfinite(p.ux) |
+                   !isfinite(p.uy) | !isfinite(p.uz) | zero_kill | (db < 0.0f) | (p.nstep >= kMaxSteps);
   if (bad) {
     if (COUNTERS) {
       atomicAdd(&tot[kKilled], 1ull);
@@ -735,13 +754,14 @@ __host__ __device__ inline size_t lds_bytes(int tables, int types, bool layered)
   return layered ? 0 : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
 }
 
-// Occupancy.  The uniform-illumination voxel-grid variants without counters or records fit 64
-// VGPRs without scratch, and 8 waves per SIMD measured 0.9% faster on the 256x256x128 layered-skin
-// workload.  The other variants keep the compiler's choice: at 8 waves the pencil-beam variant
-// measured 3% slower, and the counter, record and parametric variants spill to scratch.
+// Occupancy.  The uniform-illumination voxel-grid variant without counters, records or radial tally
+// fits 64 VGPRs without scratch, and 8 waves per SIMD measured 0.9% faster on the 256x256x128
+// layered-skin workload.  The other variants keep the compiler's choice: at 8 waves the pencil-beam
+// variant measured 3% slower, and the counter, record, radial and parametric variants spill to
+// scratch.
 template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD, bool LAYERED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
-    (UNIFORM && !COUNTERS && !RECORD && !LAYERED) ? 8 : 1, 8))) void transport_kernel(KArgs a) {
+    (UNIFORM && !COUNTERS && !RECORD && !LAYERED && !RADIAL) ? 8 : 1, 8))) void transport_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   // Voxel-grid runs (few tables, many packets each) keep the optical-property rows, launch
   // weights and the workgroup's per-table totals in LDS; the totals are accumulated with LDS
@@ -874,14 +894,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     }
     if (__ballot(alive) == 0) break;
     // The plain crossings of the flight ahead, inside the anchor's safe box, in one jump.
+    const uint32_t rbase = __umul24(table, (uint32_t)a.num_types);  // first row of the lane's table
     if (a.jump_on && alive) {
-      const float4 r1 = rows[(size_t)table * a.num_types + p.t].r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
+      const float4 r1 = row_at(rows, rbase + p.t).r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
       jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, s_skip, acc);
     }
     if (alive) {
-      const RowConst *tab = rows + (size_t)table * a.num_types;
       unsigned long long *tt = totals + (size_t)table * kTotFields;
-      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, tab, tt, acc, &geo, table, nrow);
+      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow);
       if (res != kLive) {  // tallied in the next launch phase
         alive = false;
         ended = true;
@@ -1213,6 +1233,8 @@ ymc_status ymc_engine_set_optprops(ymc_engine *e, const ymc_optprops *p) {
   if (!p || p->num_types < 2 || p->num_tables < 1 || !p->mua || !p->mus || !p->g || !p->n)
     return fail(YMC_ERR_ARG, "ymc_engine_set_optprops: need >= 2 rows (ambient + tissue) and >= 1 table");
   const int T = p->num_types, L = p->num_tables;
+  if ((size_t)T * L >= ((size_t)1 << 24))  // the kernel addresses rows with 24-bit multiplies
+    return fail(YMC_ERR_ARG, "ymc_engine_set_optprops: %d tables x %d rows exceed 2^24 rows", L, T);
   std::vector<RowConst> rows((size_t)T * L);
   std::vector<float> w0(L);
   for (int w = 0; w < L; ++w) {
