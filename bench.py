@@ -195,7 +195,7 @@ def transport(workload, photons, args, steps, warmup, absorption=False, pool_lan
             "spectra_per_s": steps / elapsed if tables > 1 else None,
             "absorption_grid": bool(absorption),
             "launch_params": {k: params[k] for k in ("pool_lanes", "jump_max_crossings", "relaunch_lanes",
-                                                      "chain_lanes") if k in params},
+                                                      "relaunch_wait", "chain_lanes") if k in params},
             "steps_per_photon": n_steps / n_photons,
             "fresnel_per_photon": fresnel / n_photons,
             "kernel_ms": kms,

@@ -1,11 +1,12 @@
 """Autotuner for the persistent transport launch.
 
-Searches, on a sample budget of the actual workload:
-  pool_lanes        resident lanes (workgroups x 256), up to full occupancy
+Searches, on the budget of the actual workload (or a stated sample of it):
+  pool_lanes        resident lanes (workgroups x 256), as a fraction of full occupancy
   jump_max_crossings  most plain crossings one safe-box flight jump may take
   relaunch_lanes    a lane whose packet ended waits until this many lanes of its wave wait, then
                     the wave tallies and relaunches them together (the phase costs the wave as much
                     for one lane as for many; waiting lanes do nothing)
+  relaunch_wait     ... or until a lane has waited this many loop iterations
   chain_lanes       the compaction interval of the persistent pool.  A lane runs a chain's packets
                     back to back, then returns to the work queue for the next chain; the guided
                     schedule gives the first chains ceil(N / (2 chain_lanes)) packets and shrinks
@@ -14,17 +15,19 @@ Searches, on a sample budget of the actual workload:
                     N is, so chain_lanes is searched as a multiple of the full-occupancy pool: short
                     chains keep lanes busy when packet costs vary between tables, long chains cost
                     fewer queue visits and stream seedings.
-pool_lanes and relaunch_lanes do not change results (tallies are bit-identical for any value,
-which the GPU tests check for the pool); jump_max_crossings changes only which crossings are jumped, i.e. float rounding of the
-positions, with the same draws; chain_lanes decides which stream each packet draws from, so results change at
-the statistical level and stay reproducible for the stored setting.  The best setting goes to a
-JSON file keyed by device name and workload signature (which includes the decade of the packet budget);
-engine runs pick it up through `tuned(...)`.
+The search is coordinate descent: from the engine defaults, each parameter in turn is swept over
+its candidate values with the others fixed and the fastest kept (a change must gain more than
+`min_gain` to count, above run-to-run noise); sweeps repeat until one changes nothing.
+pool_lanes, relaunch_lanes and relaunch_wait do not change results (tallies are bit-identical for
+any value, which the GPU tests check); jump_max_crossings changes only which crossings are jumped,
+i.e. float rounding of the positions, with the same draws; chain_lanes decides which stream each
+packet draws from, so results change at the statistical level and stay reproducible for the stored
+setting.  The best setting goes to a JSON file keyed by device name and workload signature (which
+includes the decade of the packet budget); engine runs pick it up through `tuned(...)`.
 """
 from __future__ import annotations
 
 import hashlib
-import itertools
 import json
 import os
 import time
@@ -33,12 +36,15 @@ import numpy as np
 
 CACHE = os.environ.get("YMC3D_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                           "autotune.json"))
+# Candidate values per parameter; DEFAULTS is where the descent starts (the engine's defaults).
 SEARCH = {
-    "pool_frac": (1.0, 0.5),
-    "jump_max_crossings": (16, 64),
-    "relaunch_lanes": (8, 12, 16),
-    "chain_factor": (1, 4, 16),
+    "pool_frac": (0.25, 0.375, 0.5, 0.625, 0.75, 0.875, 1.0),
+    "jump_max_crossings": (8, 16, 24, 32, 48, 63),
+    "relaunch_lanes": (4, 6, 8, 10, 12, 14, 16, 20, 24),
+    "relaunch_wait": (8, 12, 16, 24, 32),
+    "chain_factor": (0.5, 1, 2, 4, 8, 16, 32),
 }
+DEFAULTS = {"pool_frac": 1.0, "jump_max_crossings": 63, "relaunch_lanes": 12, "relaunch_wait": 16, "chain_factor": 1}
 
 
 def budget_decade(params: dict) -> int:
@@ -82,20 +88,30 @@ def device_name() -> str:
     return "gpu"
 
 
-def _load() -> dict:
-    if os.path.exists(CACHE):
-        with open(CACHE) as f:
+def _load(path: str | None = None) -> dict:
+    path = path or CACHE
+    if os.path.exists(path):
+        with open(path) as f:
             return json.load(f)
     return {}
 
 
-def tuned(scene, params: dict) -> dict:
+def tuned(scene, params: dict, cache: str | None = None) -> dict:
     """Launch parameters stored for this workload (empty if never tuned)."""
-    return dict(_load().get(device_name(), {}).get(signature(scene, params), {}).get("params", {}))
+    return dict(_load(cache).get(device_name(), {}).get(signature(scene, params), {}).get("params", {}))
 
 
-def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, verbose: bool = True) -> dict:
-    """Times every combination of SEARCH on a sample budget; stores and returns the best."""
+def _candidate(setting: dict, full: int) -> dict:
+    return {"pool_lanes": max(256, int(full * setting["pool_frac"]) // 256 * 256),
+            "jump_max_crossings": int(setting["jump_max_crossings"]), "relaunch_lanes": int(setting["relaunch_lanes"]),
+            "relaunch_wait": int(setting["relaunch_wait"]), "chain_lanes": max(256, int(full * setting["chain_factor"]))}
+
+
+def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, verbose: bool = True,
+         min_gain: float = 0.003, max_sweeps: int = 3, cache: str | None = None) -> dict:
+    """Coordinate descent over SEARCH from DEFAULTS (see the module docstring); each candidate is timed
+    `repeats` times and its fastest run counts.  Stores the best in `cache` (default CACHE) and returns
+    it, with the timings under "trials"."""
     params = dict(scene.params)
     if sample_photons:
         if "budgets" in params:
@@ -107,27 +123,43 @@ def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, ver
     engine.prepare(**params)
     engine.run()
     full = engine.pool_lanes()
-    results = []
-    for pf, jc, rl, cf in itertools.product(*SEARCH.values()):
-        cand = {"pool_lanes": max(256, int(full * pf) // 256 * 256), "jump_max_crossings": jc,
-                "relaunch_lanes": rl, "chain_lanes": int(full * cf)}
-        engine.prepare(**params, **cand)
-        ms = min(engine.run() for _ in range(repeats))
-        results.append((ms, cand))
-        if verbose:
-            print(json.dumps({"kernel_ms": round(ms, 3), **cand}), flush=True)
-    results.sort(key=lambda r: r[0])
-    best_ms, best = results[0]
-    db = _load()
+    timed = {}
+
+    def time_setting(setting):
+        cand = _candidate(setting, full)
+        key = json.dumps(cand, sort_keys=True)
+        if key not in timed:
+            engine.prepare(**params, **cand)
+            timed[key] = (min(engine.run() for _ in range(repeats)), cand)
+            if verbose:
+                print(json.dumps({"kernel_ms": round(timed[key][0], 3), **cand}), flush=True)
+        return timed[key][0]
+
+    best = dict(DEFAULTS)
+    best_ms = time_setting(best)
+    for _ in range(max_sweeps):
+        changed = False
+        for name, values in SEARCH.items():
+            for v in values:
+                trial = dict(best, **{name: v})
+                ms = time_setting(trial)
+                if ms < best_ms * (1 - min_gain):
+                    best, best_ms, changed = trial, ms, True
+        if not changed:
+            break
+    worst_ms = max(t for t, _ in timed.values())
+    chosen = _candidate(best, full)
+    path = cache or CACHE
+    db = _load(path)
     entry = db.setdefault(device_name(), {})
-    entry[signature(scene, scene.params)] = {"params": best, "kernel_ms": best_ms, "scene": scene.name,
+    entry[signature(scene, scene.params)] = {"params": chosen, "kernel_ms": best_ms, "scene": scene.name,
                                              "sample_photons": sample_photons, "tuned_at": time.time(),
-                                             "worst_ms": results[-1][0]}
-    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
-    with open(CACHE + ".tmp", "w") as f:
+                                             "worst_ms": worst_ms, "trials": len(timed)}
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    with open(path + ".tmp", "w") as f:
         json.dump(db, f, indent=1)
-    os.replace(CACHE + ".tmp", CACHE)
-    return best
+    os.replace(path + ".tmp", path)
+    return dict(chosen, kernel_ms=best_ms, worst_ms=worst_ms, trials=len(timed))
 
 
 if __name__ == "__main__":
