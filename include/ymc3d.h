@@ -102,8 +102,9 @@ typedef struct ymc_optprops {
 
 /* Run parameters.  Zero-initialise and set what you need; zeros select the defaults. */
 typedef struct ymc_params {
-  uint64_t num_photons;        /* packets per table (used when budgets == NULL) */
-  const uint64_t *budgets;     /* optional [num_tables] packet budgets */
+  uint64_t num_photons;        /* packets per table (used when budgets == NULL); at most 2^31 per table,
+                                  beyond which the 64-bit weight sums could wrap (YMC_ERR_ARG) */
+  const uint64_t *budgets;     /* optional [num_tables] packet budgets, each at most 2^31 */
   uint64_t seed;               /* init_RNG seed; 0 selects 1 */
   const volatile int32_t *cancel; /* optional host flag polled by the device; nonzero stops launches */
   int32_t illumination;        /* YMC_ILLUM_UNIFORM (src/mc3d_gpu.cu:88-89) or _PENCIL (:92-94) */

@@ -112,3 +112,25 @@ def test_pencil_outside_the_grid(engine, multipliers):
     assert int(g.totals[0]["killed"]) == int(r.totals[0]["killed"]) == 5000
     _energy_exact(g, s)
     _close(g, r)
+
+
+def test_table_budget_limit(engine):
+    """A table's packet budget is limited to 2^31, below which its 64-bit fixed-point weight sums and
+    32-bit bin counts cannot wrap; larger budgets are rejected at prepare, with any chain list."""
+    import ymc3d_amd as y
+    from ymc3d_amd import configs
+
+    scene = configs.skin_slab_sphere(photons=1)
+    scene.load(engine)
+    with pytest.raises(y.YmcError) as ei:
+        engine.prepare(**dict(scene.params, num_photons=(1 << 31) + 1))
+    assert ei.value.status == y._abi.ERR_ARG
+    chains = np.zeros(2, dtype=y._abi.CHAIN_DTYPE)
+    chains["count"] = [1 << 30, (1 << 30) + 1]
+    x, a = y.seed_streams(y.default_multipliers(), 3, 2)
+    params = {k: v for k, v in scene.params.items() if k != "num_photons"}
+    with pytest.raises(y.YmcError) as ei:
+        engine.prepare_chains(chains, x, a, **params)
+    assert ei.value.status == y._abi.ERR_ARG
+    chains["count"] = [1 << 30, 1 << 30]  # exactly 2^31: accepted
+    engine.prepare_chains(chains, x, a, **params)

@@ -1063,6 +1063,8 @@ static float smallest_float_ge(double v) {
 
 // Largest absorption grid an engine allocates (tables x voxels x 8 bytes).
 constexpr size_t kMaxAbsorptionBytes = size_t(32) << 30;
+// Largest packet budget of one table (see upload_chains).
+constexpr uint64_t kMaxTablePackets = uint64_t(1) << 31;
 
 static ymc_status check_engine(ymc_engine *e) {
   if (!e) return fail(YMC_ERR_ARG, "null engine");
@@ -1311,12 +1313,20 @@ static ymc_status upload_chains(ymc_engine *e, const ymc_chain *chains, const ui
   HIP_TRY(hipMemcpy(e->chain_x.p, x, n * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(e->chain_a.p, a, n * 4, hipMemcpyHostToDevice));
   uint64_t total = 0;
-  std::vector<uint64_t> first(n);
+  std::vector<uint64_t> first(n), per_table(e->num_tables, 0);
   for (uint64_t c = 0; c < n; ++c) {
     if (chains[c].table >= (uint32_t)e->num_tables) return fail(YMC_ERR_ARG, "chain %llu: table out of range", (unsigned long long)c);
     first[c] = total;
     total += chains[c].count;
+    per_table[chains[c].table] += chains[c].count;
   }
+  // A packet's weight never exceeds 1 (2^32 in fixed point) and its absorbed weight exceeds its
+  // launch weight only by roulette gains of under 1e-3 each, so the u64 weight sums and the u32
+  // per-bin counts of a table cannot wrap below this many packets.
+  for (int w = 0; w < e->num_tables; ++w)
+    if (per_table[w] > kMaxTablePackets)
+      return fail(YMC_ERR_ARG, "table %d: %llu packets exceed the %llu a table's 64-bit tallies hold", w,
+                  (unsigned long long)per_table[w], (unsigned long long)kMaxTablePackets);
   if (e->params.flags & YMC_FLAG_RECORD) {
     if ((st = e->chain_first.ensure(n * 8)) || (st = e->records.ensure(total * sizeof(ymc_photon_record)))) return st;
     HIP_TRY(hipMemcpy(e->chain_first.p, first.data(), n * 8, hipMemcpyHostToDevice));
@@ -1343,7 +1353,12 @@ ymc_status ymc_engine_prepare(ymc_engine *e, const ymc_params *prm) {
   if ((st = prepare_common(e, prm))) return st;
   const int L = e->num_tables;
   std::vector<uint64_t> budgets(L);
-  for (int w = 0; w < L; ++w) budgets[w] = prm->budgets ? prm->budgets[w] : prm->num_photons;
+  for (int w = 0; w < L; ++w) {
+    budgets[w] = prm->budgets ? prm->budgets[w] : prm->num_photons;
+    if (budgets[w] > kMaxTablePackets)  // checked again per chain list in upload_chains
+      return fail(YMC_ERR_ARG, "table %d: budget of %llu packets exceeds %llu", w, (unsigned long long)budgets[w],
+                  (unsigned long long)kMaxTablePackets);
+  }
   const int lanes = prm->chain_lanes > 0 ? prm->chain_lanes : 524288;
   uint64_t n = 0;
   if ((st = build_chains(budgets.data(), L, lanes, nullptr, 0, &n))) return st;
