@@ -104,6 +104,36 @@ def test_lockstep_packets(engine, multipliers, scene_fn, n):
     assert np.quantile(pos, 0.99) < 1e-3
 
 
+@pytest.mark.parametrize("flags", [y.FLAG_RECORD, y.FLAG_RECORD | y.FLAG_COUNTERS])
+def test_totals_equal_the_records(engine, multipliers, flags):
+    """The reflectance and transmittance totals of a run (fixed-point sums of W and W^2, counts)
+    equal their recomputation in exact integers from the same run's per-packet records: f =
+    RNE(W 2^32), f^2 term (f^2 + 2^31) >> 32, reflected when z <= 0, nothing for zero weight.  Pins
+    the device's fixed-point conversion and its square, which the statistical comparisons cannot."""
+    n = 20000
+    scene = configs.skin_vessels(photons=n)
+    scene.load(engine)
+    chains, x, a = _chains_one_per_packet(multipliers, n, 4242)
+    params = {k: v for k, v in scene.params.items() if k not in ("num_photons", "budgets")}
+    engine.prepare_chains(chains, x, a, flags=flags, **params)
+    engine.run()
+    rec = engine.read_records(n)
+    t = engine.read_tally().totals[0]
+    f = np.rint(rec["w"].astype(np.float64) * 2.0 ** 32).astype(np.uint64)
+    assert f.max() <= 2 ** 32 and (rec["w"] > 0).sum() > n // 2
+    want = {"R_w": 0, "R_w2": 0, "R_n": 0, "T_w": 0, "T_w2": 0, "T_n": 0}
+    for fi, z in zip(f.tolist(), rec["z"].tolist()):
+        if fi == 0:
+            continue
+        side = "R" if z <= 0.0 else "T"
+        want[side + "_w"] += fi
+        want[side + "_w2"] += (fi * fi + (1 << 31)) >> 32
+        want[side + "_n"] += 1
+    for k, v in want.items():
+        assert int(t[k]) == v, (k, int(t[k]), v)
+    assert int(t["photons"]) == n
+
+
 def _compare_maps(gres, rres, min_count=50):
     Rg, Rc = gres.drs()[0], rres.drs()[0]
     se = np.hypot(gres.drs_se()[0], rres.drs_se()[0])

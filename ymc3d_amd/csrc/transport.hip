@@ -164,13 +164,14 @@ __device__ __forceinline__ uint64_t fx(float w) {
   float s = rintf(w * 0x1p32f);
   return s > 0.0f ? (uint64_t)s : 0ull;
 }
+// (f^2 + 2^31) >> 32 for f < 2^33 (weights below 2; packet weights never exceed 1).  With f =
+// fh 2^32 + fl, fh <= 1: (f^2 + 2^31) >> 32 = fh (2^32 + 2 fl) + ((fl^2 + 2^31) >> 32) exactly, since
+// the other terms of f^2 are multiples of 2^32; fl^2 + 2^31 < 2^64 is one 32x32 -> 64-bit
+// multiply-add instead of a 128-bit product.
 __device__ __forceinline__ uint64_t fx_sq(uint64_t f) {
-  // (f*f + 2^31) >> 32 with a 128-bit intermediate
-  uint64_t lo = f * f;
-  uint64_t hi = __umul64hi(f, f);
-  uint64_t lo2 = lo + (1ull << 31);
-  hi += (lo2 < lo);
-  return (hi << 32) | (lo2 >> 32);
+  const uint32_t fl = (uint32_t)f;
+  const uint64_t low = ((uint64_t)fl * fl + (1ull << 31)) >> 32;
+  return (f >> 32) ? low + (1ull << 32) + 2ull * fl : low;
 }
 
 // photontrack_get_grid_coord (src/mc3d_gpu.cu:53-63), IEEE division; used at launch and tally.
@@ -303,10 +304,10 @@ __device__ __forceinline__ void set_anchor(Packet &p, const KArgs &a, int i, int
   p.ok = box ? zup - k : kNoCube;
   p.ext = (uint32_t)(2 * rxy) | ((uint32_t)(zup + zdn) << 16);
 }
-__device__ __forceinline__ bool in_box(const Packet &p, int i, int j, int k) {
-  const uint32_t rxy2 = p.ext & 0xffffu, rz2 = p.ext >> 16;
-  return ((uint32_t)(i + p.oi) <= rxy2) & ((uint32_t)(j + p.oj) <= rxy2) & ((uint32_t)(k + p.ok) <= rz2);
-}
+// The packet's voxel lies in the anchor's box whenever the anchor has one: every commit of an index
+// either makes the new voxel the anchor (launch, general step) or stays inside the box (jump), and a
+// step that commits no index keeps both.  So "in the box" is "has a box".
+__device__ __forceinline__ bool has_box(const Packet &p) { return p.oi != kNoCube; }
 
 // Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
 // launch boundary (src/mc3d_gpu.cu:499-504), so its end-of-launch zero-step check (:343) sees a
@@ -470,7 +471,7 @@ __device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra
   // one predicate without short-circuit branches
   const int cap = skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
   const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= cap) &
-                  !on_face & in_box(p, p.i, p.j, p.k) & axis_ok(qx, lox, hix, upx) &
+                  !on_face & has_box(p) & axis_ok(qx, lox, hix, upx) &
                   axis_ok(qy, loy, hiy, upy) & axis_ok(qz, loz, hiz, upz) & (qx > 0.0f) & (qx < a.Lx) &
                   (qy > 0.0f) & (qy < a.Ly) & (dbm <= p.s) & (p.nstep + (uint32_t)n < kMaxSteps);
   if (ok) {
@@ -503,7 +504,7 @@ enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
 
 
 // One step of a live packet.
-template <bool COUNTERS, bool LAYERED>
+template <bool COUNTERS, bool LAYERED, bool RECORD>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
                                             const RowConst *rows, uint32_t rbase, unsigned long long *tot,
                                             LaneAcc &acc, const ymc_layered *lg, uint32_t table, float4 nrow) {
@@ -607,7 +608,14 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
       }
       if (W < a.w_thresh && mu_t != 0.0f) {  // roulette (:275-277)
         const float e = u32_co(u_roul);
-        const float Wr = ((double)e < YMC_SURVIVAL_CHANCE) ? (float)((double)W / YMC_SURVIVAL_CHANCE) : 0.0f;
+        // The reference compares and divides in binary64: e < 0.1 and (float)(W / 0.1).  The
+        // binary64 constant is 0.1 (1 + 2^-54) exactly, so W / 0.1 = 10 W (1 - 2^-54 + ...), within
+        // half an ulp of 10 W, which binary64 holds exactly (W has 24 significant bits): the quotient
+        // rounds to 10 W and the float result is RN(10 W) = W * 10.0f.  No float lies in
+        // [0.1, 0.1f), so e < 0.1 <=> e < 0.1f.  Both identities are checked over every float of
+        // their range on the CPU.
+        static_assert(YMC_SURVIVAL_CHANCE == 0.1, "the binary32 roulette below assumes a chance of 0.1");
+        const float Wr = (e < 0.1f) ? W * 10.0f : 0.0f;
         if (COUNTERS) atomicAdd(&tot[kRoul], (unsigned long long)((long long)fx(Wr) - (long long)fx(W)));
         W = Wr;
       }
@@ -658,15 +666,19 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.y = ny;
     p.z = nz;
   }
-  // Mirror walls (:113-126, :330), with the reference's short-circuit order.
-  if (!refl) {
-    if (p.x >= a.Lx || p.x <= 0.0f) {
+  // Mirror walls (:113-126, :330), with the reference's short-circuit order.  Wall hits are rare, so
+  // the wave skips the block unless one of its lanes hits (the compiler otherwise issues both folds,
+  // masked, in every step).
+  const bool wall_x = (p.x >= a.Lx) | (p.x <= 0.0f);
+  const bool wall_y = (p.y >= a.Ly) | (p.y <= 0.0f);
+  if (__builtin_amdgcn_ballot_w64(!refl & (wall_x | wall_y)) != 0 && !refl) {
+    if (wall_x) {
       p.x = (p.x >= a.Lx) ? 2.0f * a.Lx - p.x : p.x;
       p.x = (p.x <= 0.0f) ? fabsf(p.x) : p.x;
       p.ux = -p.ux;
       p.iux = -p.iux;
       refl = true;
-    } else if (p.y >= a.Ly || p.y <= 0.0f) {
+    } else if (wall_y) {
       p.y = (p.y >= a.Ly) ? 2.0f * a.Ly - p.y : p.y;
       p.y = (p.y <= 0.0f) ? fabsf(p.y) : p.y;
       p.uy = -p.uy;
@@ -701,11 +713,15 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
       atomicAdd(&tot[kKilled], 1ull);
       atomicAdd(&tot[kKilledW], (unsigned long long)fx(p.W));
     }
-    p.x = p.y = p.z = 0.0f;
-    p.ux = p.uy = 0.0f;
-    p.uz = 1.0f;
+    // A dead packet tallies nothing (zero weight); its position is cleared for the record only.
+    // Clearing the other fields unconditionally cost a select each in every iteration.
+    if (RECORD) {
+      p.x = p.y = p.z = 0.0f;
+      p.ux = p.uy = 0.0f;
+      p.uz = 1.0f;
+      p.s = 0.0f;
+    }
     p.W = 0.0f;
-    p.s = 0.0f;
     return kDead;
   }
   return freeze ? kFrozen : kLive;
@@ -789,6 +805,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
   uint64_t rx = 0;
   uint32_t ra = 0;
   uint32_t left = 0, table = 0, done_in_chain = 0;
+  uint32_t rbase = 0;  // first row of the lane's table, set with each chain
   unsigned long long chain = 0;
   bool alive = false, drained = false, stop = false;
   bool ended = false;  // the lane's packet has ended and waits for its tally
@@ -873,6 +890,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
           const ymc_chain ch = a.chains[c];
           left = ch.count;
           table = ch.table;
+          rbase = __umul24(table, (uint32_t)a.num_types);
           rx = a.chain_x[c];
           ra = a.chain_a[c];
           if (LAYERED) {
@@ -894,14 +912,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     }
     if (__ballot(alive) == 0) break;
     // The plain crossings of the flight ahead, inside the anchor's safe box, in one jump.
-    const uint32_t rbase = __umul24(table, (uint32_t)a.num_types);  // first row of the lane's table
     if (a.jump_on && alive) {
       const float4 r1 = row_at(rows, rbase + p.t).r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
       jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, s_skip, acc);
     }
     if (alive) {
       unsigned long long *tt = totals + (size_t)table * kTotFields;
-      const int res = step_packet<COUNTERS, LAYERED>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow);
+      const int res = step_packet<COUNTERS, LAYERED, RECORD>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow);
       if (res != kLive) {  // tallied in the next launch phase
         alive = false;
         ended = true;
