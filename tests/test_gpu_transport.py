@@ -57,9 +57,11 @@ def test_draw_conversion_edges(engine, multipliers):
 
 
 def test_stream_skip_bit_exact(engine, multipliers):
-    """A flight jump skips the draws of n crossings with three modular products (Montgomery, one
-    base-4 digit of n each); the states must equal those of 4n draws taken one by one, for every n a
-    jump can take and for extreme states."""
+    """A flight jump skips the draws of n crossings by stepping n mod 4 crossings and taking up to two
+    modular products (Montgomery, for the multiples of 4 below 32 and for 32); the states must equal
+    those of 4n draws taken one by one, for every n a jump can take and for extreme states (the
+    largest states p - 1 and p - 2, and the state 1).  States that are not MWC states of their
+    multiplier are refused."""
     x, a = y.seed_streams(multipliers, 4711, 1024)
     p = (a.astype(np.uint64) << np.uint64(32)) - np.uint64(1)
     x[:4] = [1, (int(a[1]) - 2) << 32 | 0xFFFFFFFE, int(p[2]) - 1, (int(a[3]) - 1) << 32 | 0xFFFFFFFD]
@@ -67,6 +69,9 @@ def test_stream_skip_bit_exact(engine, multipliers):
         got = engine.rng_skip(x, a, n)
         _, want = O.mwc_fill(x, a, 4 * n, 0)
         assert np.array_equal(got, want), n
+    for bad_x, bad_a in ((p[:1], a[:1]), (x[:1], np.ones(1, np.uint32))):
+        with pytest.raises(y.YmcError):
+            engine.rng_skip(bad_x, bad_a, 1)
 
 
 def _chains_one_per_packet(multipliers, n, seed):

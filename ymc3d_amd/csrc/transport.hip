@@ -99,17 +99,14 @@ __device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t 
   const uint64_t s1 = (z23 << 32) | z1;
   const uint64_t w_lo = (uint64_t)a * z0 + s1;
   const uint32_t w_hi = (uint32_t)(z23 >> 32) + (w_lo < s1 ? 1u : 0u);
-  // second: u = a*(w mod 2^32) + w/2^32 (65 bits: c:u), u < 3p
+  // second: u = a*(w mod 2^32) + w/2^32 (65 bits: c:u).  Bound: w < p^2/2^32 + p + 1, and
+  // a*(w mod 2^32) <= p + 1 - a, so u < p^2/2^64 + (p + 1)/2^32 + p + 1 - a; with p^2/2^64 < p - p/2^32
+  // (a < 2^32) that is u < 2p + 1 + 2^-32 - a < 2p, since every multiplier has a >= 2.  One
+  // conditional subtraction makes the result canonical (the 65th bit set means u >= 2^64 > p).
   const uint64_t s2 = ((uint64_t)w_hi << 32) | (w_lo >> 32);
-  uint64_t u = (uint64_t)a * (uint32_t)w_lo + s2;
-  bool c = u < s2;
-  for (int r = 0; r < 2; ++r) {
-    if (c || u >= p) {
-      c = c && (u >= p);
-      u -= p;
-    }
-  }
-  return u;
+  const uint64_t u = (uint64_t)a * (uint32_t)w_lo + s2;
+  const bool c = u < s2;
+  return (c || u >= p) ? u - p : u;
 }
 
 // Per-lane skip table: entry k - 1 (k = 1..7) holds E(4k) and entry 7 holds E(32), where E(n) =
@@ -279,7 +276,7 @@ struct Packet {
   // [ak-zup, ak+zdn], stored as oi = rxy - ai, oj = rxy - aj, ok = zup - ak and the extents
   // rxy2 = 2 rxy, rz2 = zup + zdn, so that voxel (i, j, k) lies in the box iff
   // (unsigned)(i + oi) <= rxy2, (unsigned)(j + oj) <= rxy2 and (unsigned)(k + ok) <= rz2.  No box:
-  // offsets of -2^30, which no index passes (extents <= 510).  The two extents share one word
+  // oi = -2^30 (kNoCube), which no index passes (extents <= 510).  The two extents share one word
   // (rxy2 | rz2 << 16): as two fields the compiler kept them in a private array in scratch.
   int oi, oj, ok;
   uint32_t ext;
@@ -290,7 +287,7 @@ constexpr int kNoCube = -(1 << 30);
 // Voxel word fields, in the layout the kernel arguments name (bit-field extracts, no repacking).
 __device__ __forceinline__ int word_row(const KArgs &a, uint32_t w) { return (int)(w & a.w_row); }
 
-// Voxel (i, j, k) with word w becomes the anchor; with box false the packet has no box (offsets
+// Voxel (i, j, k) with word w becomes the anchor; with box false the packet has no box (an x offset
 // that no index passes, whatever the extents hold).  Every field is written on both paths, with
 // selected values: a branch that wrote some fields on one side only led the compiler to keep the
 // box fields in scratch.
@@ -299,9 +296,9 @@ __device__ __forceinline__ void set_anchor(Packet &p, const KArgs &a, int i, int
   const int zup = (int)__builtin_amdgcn_ubfe(w, a.w_zup, a.w_zbits);
   const int zdn = (int)__builtin_amdgcn_ubfe(w, a.w_zdn, a.w_zbits);
   const int rxy = (int)(w >> a.w_rxy);
-  p.oi = box ? rxy - i : kNoCube;
-  p.oj = box ? rxy - j : kNoCube;
-  p.ok = box ? zup - k : kNoCube;
+  p.oi = box ? rxy - i : kNoCube;  // the sentinel in oi alone marks "no box" (has_box)
+  p.oj = rxy - j;
+  p.ok = zup - k;
   p.ext = (uint32_t)(2 * rxy) | ((uint32_t)(zup + zdn) << 16);
 }
 // The packet's voxel lies in the anchor's box whenever the anchor has one: every commit of an index
@@ -1114,6 +1111,8 @@ extern "C" {
 
 ymc_status ymc_engine_create(int32_t device, const uint32_t *table, uint32_t table_len, ymc_engine **out) {
   if (!out || !table || table_len < 2) return fail(YMC_ERR_ARG, "ymc_engine_create: need a multiplier table of >= 2 entries");
+  for (uint32_t q = 0; q < table_len; ++q)
+    if (table[q] < 2) return fail(YMC_ERR_ARG, "ymc_engine_create: multiplier %u at %u is below 2", table[q], q);
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(YMC_ERR_DEVICE, "no HIP device available");
@@ -1393,6 +1392,12 @@ ymc_status ymc_engine_prepare_chains(ymc_engine *e, const ymc_params *prm, const
   ymc_status st = check_engine(e);
   if (st) return st;
   if (!chains || !x || !a) return fail(YMC_ERR_ARG, "ymc_engine_prepare_chains: null chain arrays");
+  // Streams must be MWC states of their multiplier: a >= 2 and carry*2^32 + value < a*2^32 - 1 (the
+  // stream skip's modular arithmetic relies on both).
+  for (uint64_t c = 0; c < n_chains; ++c)
+    if (a[c] < 2 || x[c] >= (((uint64_t)a[c] << 32) - 1))
+      return fail(YMC_ERR_ARG, "chain %llu: state 0x%llx is not an MWC state of multiplier %u", (unsigned long long)c,
+                  (unsigned long long)x[c], a[c]);
   if ((st = prepare_common(e, prm))) return st;
   return upload_chains(e, chains, x, a, n_chains);
 }
@@ -1632,6 +1637,9 @@ ymc_status ymc_engine_rng_skip(ymc_engine *e, const uint64_t *x, const uint32_t 
   if (st) return st;
   if (!x || !a || !x_final || n_streams == 0) return fail(YMC_ERR_ARG, "ymc_engine_rng_skip: bad arguments");
   if (crossings > (uint32_t)kSkipMax) return fail(YMC_ERR_ARG, "ymc_engine_rng_skip: crossings must be <= %d", kSkipMax);
+  for (uint32_t q = 0; q < n_streams; ++q)
+    if (a[q] < 2 || x[q] >= (((uint64_t)a[q] << 32) - 1))
+      return fail(YMC_ERR_ARG, "ymc_engine_rng_skip: stream %u is not an MWC state of its multiplier", q);
   DevBuf dx, da, dxf;
   if ((st = dx.ensure((size_t)n_streams * 8)) || (st = da.ensure((size_t)n_streams * 4)) ||
       (st = dxf.ensure((size_t)n_streams * 8)))
