@@ -28,3 +28,13 @@ def test_tune_stores_the_fastest_setting(engine, tmp_path, capsys):
     assert {k: best[k] for k in keys} == stored
     res = engine.simulate(**scene.params, **stored)
     assert int(res.totals["photons"].sum()) == 200_000 and np.all(res.R_w >= 0)
+    # A second tune of the same workload also descends from the stored setting: plant one away from
+    # the defaults' path, and the planted setting is among the timed candidates.
+    planted = dict(stored, relaunch_lanes=20, relaunch_wait=8)
+    db = json.load(open(cache))
+    db[autotune.device_name()][autotune.signature(scene, scene.params)]["params"] = planted
+    json.dump(db, open(cache, "w"))
+    again = autotune.tune(engine, scene, repeats=1, max_sweeps=1, cache=cache, verbose=True)
+    timed = [json.loads(l) for l in capsys.readouterr().out.splitlines() if l.startswith("{")]
+    assert any(all(t[k] == planted[k] for k in keys) for t in timed)
+    assert again["kernel_ms"] <= min(t["kernel_ms"] for t in timed) * 1.003 + 1e-9

@@ -143,3 +143,19 @@ def test_autotune_store(tmp_path, monkeypatch):
     big = configs.skin_slab_sphere(photons=1000)
     assert autotune.signature(big, big.params) != autotune.signature(a, a.params)
     assert autotune.tuned(big, big.params) == {}
+
+
+def test_autotune_warm_start_coordinates():
+    """Stored launch parameters map back to the search coordinates the descent starts from: exact
+    for a setting on the grid, snapped to the nearest candidate otherwise, defaults for missing keys."""
+    full = 524288
+    stored = {"pool_lanes": 458752, "jump_max_crossings": 63, "relaunch_lanes": 10, "relaunch_wait": 32,
+              "chain_lanes": 1048576}
+    s = autotune._setting(stored, full)
+    assert s == {"pool_frac": 0.875, "jump_max_crossings": 63, "relaunch_lanes": 10, "relaunch_wait": 32,
+                 "chain_factor": 2}
+    assert autotune._candidate(s, full) == stored
+    off = autotune._setting({"pool_lanes": 300000, "chain_lanes": 3 * full, "relaunch_lanes": 11}, full)
+    assert off["pool_frac"] == 0.625 and off["chain_factor"] in (2, 4) and off["relaunch_lanes"] in (10, 12)
+    assert off["jump_max_crossings"] == autotune.DEFAULTS["jump_max_crossings"]
+    assert all(off[k] in autotune.SEARCH[k] for k in autotune.SEARCH)

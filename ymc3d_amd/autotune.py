@@ -17,7 +17,9 @@ Searches, on the budget of the actual workload (or a stated sample of it):
                     fewer queue visits and stream seedings.
 The search is coordinate descent: from the engine defaults, each parameter in turn is swept over
 its candidate values with the others fixed and the fastest kept (a change must gain more than
-`min_gain` to count, above run-to-run noise); sweeps repeat until one changes nothing.
+`min_gain` to count, above run-to-run noise); sweeps repeat until one changes nothing.  A workload
+tuned before is also descended from its stored setting, and the faster end point is kept: pool size
+and chain length interact, so one descent can stop at a local optimum.
 pool_lanes, relaunch_lanes and relaunch_wait do not change results (tallies are bit-identical for
 any value, which the GPU tests check); jump_max_crossings changes only which crossings are jumped,
 i.e. float rounding of the positions, with the same draws; chain_lanes decides which stream each
@@ -107,6 +109,13 @@ def _candidate(setting: dict, full: int) -> dict:
             "relaunch_wait": int(setting["relaunch_wait"]), "chain_lanes": max(256, int(full * setting["chain_factor"]))}
 
 
+def _setting(params: dict, full: int) -> dict:
+    """The search coordinates of stored launch parameters, each snapped to its nearest candidate."""
+    raw = {"pool_frac": params.get("pool_lanes", full) / full, "chain_factor": params.get("chain_lanes", full) / full,
+           **{k: params.get(k, DEFAULTS[k]) for k in ("jump_max_crossings", "relaunch_lanes", "relaunch_wait")}}
+    return {k: min(SEARCH[k], key=lambda v: abs(v - raw[k])) for k in SEARCH}
+
+
 def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, verbose: bool = True,
          min_gain: float = 0.003, max_sweeps: int = 3, cache: str | None = None) -> dict:
     """Coordinate descent over SEARCH from DEFAULTS (see the module docstring); each candidate is timed
@@ -135,18 +144,32 @@ def tune(engine, scene, sample_photons: int | None = None, repeats: int = 2, ver
                 print(json.dumps({"kernel_ms": round(timed[key][0], 3), **cand}), flush=True)
         return timed[key][0]
 
-    best = dict(DEFAULTS)
-    best_ms = time_setting(best)
-    for _ in range(max_sweeps):
-        changed = False
-        for name, values in SEARCH.items():
-            for v in values:
-                trial = dict(best, **{name: v})
-                ms = time_setting(trial)
-                if ms < best_ms * (1 - min_gain):
-                    best, best_ms, changed = trial, ms, True
-        if not changed:
-            break
+    def descend(start):
+        best = dict(start)
+        best_ms = time_setting(best)
+        for _ in range(max_sweeps):
+            changed = False
+            for name, values in SEARCH.items():
+                for v in values:
+                    trial = dict(best, **{name: v})
+                    ms = time_setting(trial)
+                    if ms < best_ms * (1 - min_gain):
+                        best, best_ms, changed = trial, ms, True
+            if not changed:
+                break
+        return best, best_ms
+
+    # Coordinate descent finds a local optimum (pool size and chain length interact), so it starts
+    # from the engine defaults and, when this workload was tuned before, from the stored setting too;
+    # the faster end point wins.
+    best, best_ms = descend(DEFAULTS)
+    prev = tuned(scene, scene.params, cache)
+    if prev:
+        start = _setting(prev, full)
+        if _candidate(start, full) != _candidate(best, full):
+            other, other_ms = descend(start)
+            if other_ms < best_ms:
+                best, best_ms = other, other_ms
     worst_ms = max(t for t, _ in timed.values())
     chosen = _candidate(best, full)
     path = cache or CACHE

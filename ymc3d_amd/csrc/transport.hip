@@ -401,7 +401,9 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
 // next to a face; the jump below checks the position it commits against the voxel it names, so any
 // voxel of the box is safe here.  The clamp is in integers (the conversion saturates; NaN gives 0).
 __device__ __forceinline__ int axis_voxel(float c, float inv_d, int lo, int hi) {
-  const int q = (int)floorf(c * inv_d);
+  // floor and conversion in one instruction (the compiler issues v_floor_f32 + v_cvt_i32_f32)
+  int q;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(q) : "v"(c * inv_d));
   return min(max(q, lo), hi);
 }
 
