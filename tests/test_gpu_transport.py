@@ -104,8 +104,13 @@ def test_lockstep_packets(engine, multipliers, scene_fn, n):
             assert same[m].mean() >= need, (lo, hi, same[m].mean())
     assert frac > 0.75
     w_ok = np.abs(g["w"][same] - r["w"][same]) <= 1e-4 * np.maximum(r["w"][same], 1e-3)
-    pos = np.abs(g["x"][same] - r["x"][same]) + np.abs(g["y"][same] - r["y"][same])
     assert w_ok.mean() > 0.99
+    # End positions of the packets that carry weight (those the tallies bin).  A packet frozen by
+    # roulette has weight zero and the device keeps the start of its last flight as its position
+    # where the oracle keeps the last face the flight crossed.
+    carried = same & (r["w"] > 0)
+    pos = np.abs(g["x"][carried] - r["x"][carried]) + np.abs(g["y"][carried] - r["y"][carried])
+    assert carried.sum() > n // 20
     assert np.quantile(pos, 0.99) < 1e-3
 
 

@@ -24,10 +24,11 @@
 //    four draws the reference consumes on every step;
 //  - plain voxel crossings inside a region of one tissue are taken together: each voxel word
 //    carries a safe box (vertical and lateral half-extents of a box of voxels that all share its
-//    tissue and lie inside the lateral walls and below the top surface), and the part of a
-//    flight that stays inside the safe box of its anchor voxel is one jump to its last face,
-//    with no voxel load and no Fresnel test, and the stream advanced by the draws of the
-//    crossings it replaces.
+//    tissue and lie inside the lateral walls and below the top surface), and a step takes the
+//    safe box of its anchor voxel as its region: the flight runs to the box's faces or to the
+//    interaction point, the crossings inside the box cost no voxel load and no Fresnel test, the
+//    stream is advanced by their draws, and the step ends as the reference's step from the last
+//    voxel of the flight.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -113,7 +114,7 @@ __device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t 
 // 2^-128n * 2^64 mod p skips the draws of n crossings (4n steps) in one mwc_mulmod.  E(4) is the
 // state 14 MWC steps after v = 1 (2^-32*14 = 2^-128*4 * 2^64); the others follow from
 // mwc_mulmod(E(a), E(b)) = E(a + b).  A skip of n = d + 4k + 32m crossings (d, k, m its digits in
-// bases 4, 8, 2) steps d crossings and takes one product for k and one for m: jumps of fewer than 32
+// bases 4, 8, 2) steps d crossings and takes one product for k and one for m: flights of fewer than 32
 // crossings, nearly all of them, need one product.  tab points at the lane's first entry; entries are
 // STRIDE apart (one column per lane of the workgroup).
 constexpr int kSkipEntries = 8;
@@ -246,8 +247,8 @@ struct KArgs {
   unsigned long long n_chains;
   const volatile int *cancel;
   int poll_interval;
-  int jump_on;   // safe-box flight jumps enabled
-  int jump_cap;  // most crossings one jump may take
+  int jump_on;   // box flights enabled (the plain crossings inside a safe box taken together)
+  int jump_cap;  // most plain crossings one box flight may take
   int skip_off;  // byte offset of the per-lane stream-skip table in dynamic LDS, or -1 (then the
                  // skipped draws are taken one by one)
   int relaunch_lanes;  // the launch phase runs when this many lanes of a wave wait for it,
@@ -302,8 +303,9 @@ __device__ __forceinline__ void set_anchor(Packet &p, const KArgs &a, int i, int
   p.ext = (uint32_t)(2 * rxy) | ((uint32_t)(zup + zdn) << 16);
 }
 // The packet's voxel lies in the anchor's box whenever the anchor has one: every commit of an index
-// either makes the new voxel the anchor (launch, general step) or stays inside the box (jump), and a
-// step that commits no index keeps both.  So "in the box" is "has a box".
+// either makes the new voxel the anchor (launch, the end of a step) or stays inside the box (the
+// crossings of a box flight), and a step that commits no index keeps both.  So "in the box" is "has a
+// box".
 __device__ __forceinline__ bool has_box(const Packet &p) { return p.oi != kNoCube; }
 
 // Steps per launch of the reference (NUM_PHOTON_STEPS, src/mc3d_types.h:16).  Packets start at a
@@ -398,8 +400,8 @@ __device__ __forceinline__ void launch_packet(Packet &p, uint64_t &rx, uint32_t 
 }
 
 // Voxel of coordinate c along one axis, clamped to [lo, hi].  Rounding can put it one voxel off
-// next to a face; the jump below checks the position it commits against the voxel it names, so any
-// voxel of the box is safe here.  The clamp is in integers (the conversion saturates; NaN gives 0).
+// next to a face; the step below checks the end point of a flight against the voxel this names, so
+// any voxel of the box is safe here.  The clamp is in integers (the conversion saturates; NaN gives 0).
 __device__ __forceinline__ int axis_voxel(float c, float inv_d, int lo, int hi) {
   // floor and conversion in one instruction (the compiler issues v_floor_f32 + v_cvt_i32_f32)
   int q;
@@ -407,129 +409,130 @@ __device__ __forceinline__ int axis_voxel(float c, float inv_d, int lo, int hi) 
   return min(max(q, lo), hi);
 }
 
-// Coordinate c in the voxel with faces lo = e*d and hi = (e+1)*d is a state the face test continues
-// exactly: inside the voxel's closed extent and strictly before the face ahead (so the next step has
-// a positive length).  Compares only, no branches.
-__device__ __forceinline__ bool axis_ok(float c, float lo, float hi, bool up) {
-  return (c >= lo) & (c <= hi) & (c != (up ? hi : lo));
-}
+// Outcome of one step.
+enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
 
-// Flight jump.  Inside the safe box of the anchor voxel every voxel has the anchor's tissue, no
-// mirror wall and no top surface, so each face the straight path crosses there is a plain
-// crossing of the general step: hit, no Fresnel draw, new indices, s -= db*mu_t, four draws.
-// The path stops at the interaction point (optical depth s) or where it leaves the box,
-// whichever comes first.  The jump moves the packet to the last face it crosses before that
-// point, in one step, and advances the stream by the four draws of each crossing it skips; the
-// general step then takes the interaction or the crossing out of the box.  The draws are skipped
-// with the lane's skip table (up to two modular products) when the kernel has one in LDS.  Positions are the
-// general step's up to rounding (one fma from the start instead of one per crossing), and the
-// draws are the same, so packets stay in lockstep with step-by-step transport.
-// Returns true and commits when the jump was taken.  Not taken: no face to cross, more than
-// jump_cap crossings, a start on a face ahead (the general step takes that zero-length step, which
-// crosses every face the packet sits on together and is subject to the zero-step kill), or a
-// rounding case the commit checks reject.  Inside the flight the step-by-step walk can also meet a
-// zero-length step, when a crossing's end position rounds onto the next face; the jump takes it
-// like the other crossings, without the zero-step kill.
-template <bool COUNTERS>
-__device__ __forceinline__ bool jump_flight(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a, float mu_t,
-                                            float inv_mu_t, const uint64_t *skip, LaneAcc &acc) {
+// One step of a live packet, with the plain crossings of its flight before it.
+//
+// The step's region is the safe box of the anchor voxel or the packet's voxel.  Inside the box every
+// voxel has the anchor's tissue, no mirror wall and no top surface, so each face the straight path
+// crosses there is a plain crossing of the reference's step: hit, no Fresnel draw, new indices,
+// s -= db*mu_t, four draws.  The flight runs to the region's faces ahead or to the interaction
+// point, whichever comes first (photontrack_intersection and the hop decision, src/mc3d_gpu.cu:29-49,
+// 196-207, on the region's faces); the plain crossings on the way are counted, the stream is advanced
+// by their draws (with the lane's skip table, up to two modular products, when the kernel has one in
+// LDS), and the step ends as the reference's step from the last voxel of the flight: the crossing out
+// of the box (next voxel, Fresnel, ...) or the interaction.  With the voxel as the region this is the
+// reference's step.
+//   Positions are the reference's up to rounding (the end point is one fma from the flight's start
+// instead of one per crossing) and the draws are the same, so packets stay in lockstep with
+// step-by-step transport.  The region is the voxel when the packet has no box, sits on a face ahead
+// (the reference takes that zero-length step on its own, subject to the zero-step kill) or the
+// medium is clear, and a box flight is redone in the voxel when it would cross more than jump_cap
+// faces or when its end point does not lie strictly inside the last voxel across every axis it does
+// not leave the box by (a face met exactly at the end would be a step of its own in the reference).
+// A zero-length crossing inside a flight (a crossing's end position rounding onto the next face) is
+// taken like the other crossings, without the zero-step kill.  A packet frozen at the end of a flight
+// (roulette) keeps the flight's start as its position, where the reference keeps the last face it
+// crossed; its weight is zero, so no tally depends on it.
+template <bool COUNTERS, bool LAYERED, bool RECORD>
+__device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
+                                            const RowConst *rows, uint32_t rbase, unsigned long long *tot,
+                                            LaneAcc &acc, const ymc_layered *lg, uint32_t table, float4 nrow,
+                                            const uint64_t *skip) {
+  // this table's rows start at rows[rbase]
+  const float4 c0 = row_at(rows, rbase + p.t).r0;
+  const float4 c1 = row_at(rows, rbase + p.t).r1;
+  const float ni = c0.w;
+  const float mu_t = c1.x;
   const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
-  // box of the anchor in index space: [lo, lo + extent] on each axis (empty when there is none)
-  const int loi = -p.oi, loj = -p.oj, lok = -p.ok;
-  const int rxy2 = (int)(p.ext & 0xffffu), rz2 = (int)(p.ext >> 16);
-  const int hii = loi + rxy2, hij = loj + rxy2, hik = lok + rz2;
-  const float tex = ((float)(upx ? hii + 1 : loi) * a.dx - p.x) * p.iux;
-  const float tey = ((float)(upy ? hij + 1 : loj) * a.dy - p.y) * p.iuy;
-  const float tez = ((float)(upz ? hik + 1 : lok) * a.dz - p.z) * p.iuz;
-  const float t_stop = fminf(fminf(tex, fminf(tey, tez)), p.s * inv_mu_t);
-  // voxel at the stop point, clamped into the box
-  const int ei = axis_voxel(fmaf(p.ux, t_stop, p.x), a.idx, loi, hii);
-  const int ej = axis_voxel(fmaf(p.uy, t_stop, p.y), a.idy, loj, hij);
-  const int ek = axis_voxel(fmaf(p.uz, t_stop, p.z), a.idz, lok, hik);
-  // faces of voxel e on each axis; the last face crossed is the one entering it
-  const float lox = (float)ei * a.dx, hix = (float)(ei + 1) * a.dx;
-  const float loy = (float)ej * a.dy, hiy = (float)(ej + 1) * a.dy;
-  const float loz = (float)ek * a.dz, hiz = (float)(ek + 1) * a.dz;
-  const float fx = upx ? lox : hix;
-  const float fy = upy ? loy : hiy;
-  const float fz = upz ? loz : hiz;
-  const float tlx = (ei != p.i) ? (fx - p.x) * p.iux : 0.0f;
-  const float tly = (ej != p.j) ? (fy - p.y) * p.iuy : 0.0f;
-  const float tlz = (ek != p.k) ? (fz - p.z) * p.iuz : 0.0f;
-  const float tl = fmaxf(tlx, fmaxf(tly, tlz));
-  const float qx = (tl == tlx) ? fx : fmaf(p.ux, tl, p.x);
-  const float qy = (tl == tly) ? fy : fmaf(p.uy, tl, p.y);
-  const float qz = (tl == tlz) ? fz : fmaf(p.uz, tl, p.z);
-  const int di = upx ? ei - p.i : p.i - ei;
-  const int dj = upy ? ej - p.j : p.j - ej;
-  const int dk = upz ? ek - p.k : p.k - ek;
-  const int n = di + dj + dk;  // crossings, one step each
-  const float dbm = mul_rn(tl, mu_t);
-  const bool on_face = (p.x == (float)(upx ? p.i + 1 : p.i) * a.dx) | (p.y == (float)(upy ? p.j + 1 : p.j) * a.dy) |
-                       (p.z == (float)(upz ? p.k + 1 : p.k) * a.dz);
-  // one predicate without short-circuit branches
-  const int cap = skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
-  const bool ok = (tl > 0.0f) & (tl <= t_stop) & (mu_t > 0.0f) & ((di | dj | dk) >= 0) & (n <= cap) &
-                  !on_face & has_box(p) & axis_ok(qx, lox, hix, upx) &
-                  axis_ok(qy, loy, hiy, upy) & axis_ok(qz, loz, hiz, upz) & (qx > 0.0f) & (qx < a.Lx) &
-                  (qy > 0.0f) & (qy < a.Ly) & (dbm <= p.s) & (p.nstep + (uint32_t)n < kMaxSteps);
-  if (ok) {
-    p.x = qx;
-    p.y = qy;
-    p.z = qz;
-    p.i = ei;
-    p.j = ej;
-    p.k = ek;
-    p.s -= dbm;
-    p.nstep += (uint32_t)n;
-    if (COUNTERS) acc.steps += (uint32_t)n;
+
+  // Faces ahead of the packet's voxel (:33-35).
+  const float vbx = (float)(upx ? p.i + 1 : p.i) * a.dx;
+  const float vby = (float)(upy ? p.j + 1 : p.j) * a.dy;
+  const float vbz = (float)(upz ? p.k + 1 : p.k) * a.dz;
+  const bool on_face = (p.x == vbx) | (p.y == vby) | (p.z == vbz);
+  // An x or y index outside the grid (a packet launched on the far wall) anchors a box that reaches
+  // past the walls: such a packet steps in its voxel.
+  const bool outside = ((uint32_t)p.i >= (uint32_t)a.nx) | ((uint32_t)p.j >= (uint32_t)a.ny);
+  bool box = a.jump_on && has_box(p) && (mu_t > 0.0f) && !on_face && !outside;
+  // skip table present (a kernel argument, so the tests below are wave-uniform)
+  const bool has_skip = a.skip_off >= 0;
+  const int cap = has_skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
+
+  float db, dbm, nx, ny, nz, bx, by, bz;
+  int ei, ej, ek, n;
+  bool hit;
+  // The flight in a region; returns false for a box flight that fails its checks.
+  auto flight = [&](const bool box) -> bool {
+    // region [lo, hi] on each axis
+    const int rxy2 = box ? (int)(p.ext & 0xffffu) : 0, rz2 = box ? (int)(p.ext >> 16) : 0;
+    const int loi = box ? -p.oi : p.i, loj = box ? -p.oj : p.j, lok = box ? -p.ok : p.k;
+    const int hii = loi + rxy2, hij = loj + rxy2, hik = lok + rz2;
+    const float rbx = box ? (float)(upx ? hii + 1 : loi) * a.dx : vbx;
+    const float rby = box ? (float)(upy ? hij + 1 : loj) * a.dy : vby;
+    const float rbz = box ? (float)(upz ? hik + 1 : lok) * a.dz : vbz;
+    // photontrack_intersection (:29-49) with cached reciprocals, on the region's faces
+    const float tx = (rbx - p.x) * p.iux;
+    const float ty = (rby - p.y) * p.iuy;
+    const float tz = (rbz - p.z) * p.iuz;
+    db = fminf(tx, fminf(ty, tz));
+    dbm = mul_rn(db, mu_t);  // rounded product (no contraction into s - db*mu_t)
+    hit = dbm <= p.s;        // :197
+    const float hop = hit ? db : p.s * c1.z;  // distance moved (:198-206)
+    const bool ex = hit && db == tx, ey = hit && db == ty, ez = hit && db == tz;  // faces reached
+    nx = ex ? rbx : fmaf(p.ux, hop, p.x);
+    ny = ey ? rby : fmaf(p.uy, hop, p.y);
+    nz = ez ? rbz : fmaf(p.uz, hop, p.z);
+    if (!box) {  // the reference's step in the packet's voxel
+      ei = p.i;
+      ej = p.j;
+      ek = p.k;
+      bx = vbx;
+      by = vby;
+      bz = vbz;
+      n = 0;
+      return true;
+    }
+    // the flight's last voxel and its faces
+    ei = axis_voxel(nx, a.idx, loi, hii);
+    ej = axis_voxel(ny, a.idy, loj, hij);
+    ek = axis_voxel(nz, a.idz, lok, hik);
+    const float lox = (float)ei * a.dx, hix = (float)(ei + 1) * a.dx;
+    const float loy = (float)ej * a.dy, hiy = (float)(ej + 1) * a.dy;
+    const float loz = (float)ek * a.dz, hiz = (float)(ek + 1) * a.dz;
+    bx = upx ? hix : lox;
+    by = upy ? hiy : loy;
+    bz = upz ? hiz : loz;
+    const int di = upx ? ei - p.i : p.i - ei;
+    const int dj = upy ? ej - p.j : p.j - ej;
+    const int dk = upz ? ek - p.k : p.k - ek;
+    n = di + dj + dk;  // plain crossings before the step
+    const bool inside = (ex | ((nx > lox) & (nx < hix))) & (ey | ((ny > loy) & (ny < hiy))) &
+                        (ez | ((nz > loz) & (nz < hiz)));
+    return inside & ((di | dj | dk) >= 0) & (n <= cap) & (p.nstep + (uint32_t)n < kMaxSteps);
+  };
+  const bool ok = flight(box);
+  if (__builtin_amdgcn_ballot_w64(!ok) != 0) {  // rare: redo failed box flights in the voxel
+    if (!ok) flight(false);
   }
-  if (skip) {
-    if (ok) skip_crossings<kBlock>(rx, ra, skip, n);
+  if (COUNTERS) acc.steps += (uint32_t)n;
+  p.nstep += (uint32_t)n;
+  p.i = ei;
+  p.j = ej;
+  p.k = ek;
+  if (has_skip) {
+    skip_crossings<kBlock>(rx, ra, skip, n);
   } else {
-    const int steps = ok ? n : 0;
-    for (int q = 0; q < steps; ++q) {
+    for (int q = 0; q < n; ++q) {
       mwc_next(rx, ra);
       mwc_next(rx, ra);
       mwc_next(rx, ra);
       mwc_next(rx, ra);
     }
   }
-  return ok;
-}
-
-// Outcome of one step.
-enum StepResult : int { kLive = 0, kFrozen = 1, kDead = 2 };
-
-
-// One step of a live packet.
-template <bool COUNTERS, bool LAYERED, bool RECORD>
-__device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
-                                            const RowConst *rows, uint32_t rbase, unsigned long long *tot,
-                                            LaneAcc &acc, const ymc_layered *lg, uint32_t table, float4 nrow) {
-  // this table's rows start at rows[rbase]
-  const float4 c0 = row_at(rows, rbase + p.t).r0;
-  const float4 c1 = row_at(rows, rbase + p.t).r1;
-  const float ni = c0.w;
-  const float mu_t = c1.x;
-
-  // photontrack_intersection (:29-49) with cached reciprocals.
-  const float bx = (float)(p.ux >= 0.0f ? p.i + 1 : p.i) * a.dx;
-  const float by = (float)(p.uy >= 0.0f ? p.j + 1 : p.j) * a.dy;
-  const float bz = (float)(p.uz >= 0.0f ? p.k + 1 : p.k) * a.dz;
-  const float tx = (bx - p.x) * p.iux;
-  const float ty = (by - p.y) * p.iuy;
-  const float tz = (bz - p.z) * p.iuz;
-  const float db = fminf(tx, fminf(ty, tz));
-
-  const float dbm = mul_rn(db, mu_t);  // rounded product (no contraction into s - db*mu_t)
-  const bool hit = dbm <= p.s;                      // :197
-  const float ds = hit ? dbm : p.s;                 // :198
-  const float hop = hit ? db : ds * c1.z;           // distance moved
-  float nx = (hit && db == tx) ? bx : fmaf(p.ux, hop, p.x);
-  float ny = (hit && db == ty) ? by : fmaf(p.uy, hop, p.y);
-  float nz = (hit && db == tz) ? bz : fmaf(p.uz, hop, p.z);
-  const float s_left = p.s - ds;                    // :207
+  const float ds = hit ? dbm : p.s;   // :198
+  const float s_left = p.s - ds;      // :207
 
   // Next voxel (:209-213): an axis steps when the new coordinate reached the face ahead.  This
   // equals the reference's grid_coord of the new position while each index names a grid voxel
@@ -539,7 +542,6 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   int ni_ = next_index(p.i, p.ux, nx, bx);
   int nj_ = next_index(p.j, p.uy, ny, by);
   int nk_ = next_index(p.k, p.uz, nz, bz);
-  const bool outside = ((uint32_t)p.i >= (uint32_t)a.nx) | ((uint32_t)p.j >= (uint32_t)a.ny);
   if (__builtin_amdgcn_ballot_w64(outside) != 0) {
     if (outside) {
       ni_ = grid_coord(nx, p.ux, a.dx);
@@ -769,14 +771,14 @@ __host__ __device__ inline size_t lds_bytes(int tables, int types, bool layered)
   return layered ? 0 : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
 }
 
-// Occupancy.  The uniform-illumination voxel-grid variant without counters, records or radial tally
-// fits 64 VGPRs without scratch, and 8 waves per SIMD measured 0.9% faster on the 256x256x128
-// layered-skin workload.  The other variants keep the compiler's choice: at 8 waves the pencil-beam
-// variant measured 3% slower, and the counter, record, radial and parametric variants spill to
-// scratch.
+// Occupancy.  The voxel-grid variants without counters, records or radial tally are held to 7 waves
+// per SIMD (72 VGPRs): held to 8 (64 VGPRs) the box-flight step keeps a value in scratch, and the
+// tuned pool of the 256x256x128 layered-skin workload runs 7 waves per SIMD either way.  The other
+// variants keep the compiler's choice: the counter, record, radial and parametric variants spill to
+// scratch at 7 waves.
 template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD, bool LAYERED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
-    (UNIFORM && !COUNTERS && !RECORD && !LAYERED && !RADIAL) ? 8 : 1, 8))) void transport_kernel(KArgs a) {
+    (!COUNTERS && !RECORD && !LAYERED && !RADIAL) ? 7 : 1, 8))) void transport_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   // Voxel-grid runs (few tables, many packets each) keep the optical-property rows, launch
   // weights and the workgroup's per-table totals in LDS; the totals are accumulated with LDS
@@ -910,14 +912,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
       }
     }
     if (__ballot(alive) == 0) break;
-    // The plain crossings of the flight ahead, inside the anchor's safe box, in one jump.
-    if (a.jump_on && alive) {
-      const float4 r1 = row_at(rows, rbase + p.t).r1;  // {mu_t, mu_a/mu_t, 1/mu_t, ...}
-      jump_flight<COUNTERS>(p, rx, ra, a, r1.x, r1.z, s_skip, acc);
-    }
     if (alive) {
       unsigned long long *tt = totals + (size_t)table * kTotFields;
-      const int res = step_packet<COUNTERS, LAYERED, RECORD>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow);
+      const int res =
+          step_packet<COUNTERS, LAYERED, RECORD>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow, s_skip);
       if (res != kLive) {  // tallied in the next launch phase
         alive = false;
         ended = true;
