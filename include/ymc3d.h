@@ -125,11 +125,12 @@ typedef struct ymc_params {
   int32_t pool_lanes;          /* resident lanes (device); 0 selects full occupancy */
   int32_t compact_interval;    /* loop iterations between polls of the cancel flag; 0 selects 1024 */
   uint32_t flags;              /* YMC_FLAG_* */
-  int32_t jump_mode;           /* safe-box flight jumps: 0 selects default (on), 1 on, 2 off.  A
-                                  flight whose faces all separate voxels of one tissue moves to
-                                  its last face at once, drawing what the plain crossings draw */
-  int32_t jump_max_crossings;  /* most crossings one jump may take; 0 selects 64.  At most 63 when
-                                  the jumped draws are skipped with the per-lane LDS table, which
+  int32_t jump_mode;           /* safe-box flights: 0 selects default (on), 1 on, 2 off.  A step
+                                  takes the plain crossings of its flight inside the anchor voxel's
+                                  safe box (faces between voxels of one tissue) together, drawing
+                                  what those crossings draw */
+  int32_t jump_max_crossings;  /* most crossings one box flight may take; 0 selects 64.  At most 63
+                                  when their draws are skipped with the per-lane LDS table, which
                                   the engine uses whenever it costs no occupancy */
   int32_t relaunch_lanes;      /* a lane whose packet ended waits until this many lanes of its wave
                                   wait (tally, next chain, launch run for all at once); 0 selects 12,
@@ -235,7 +236,7 @@ ymc_status ymc_safe_radius(const uint8_t *rows, int32_t nx, int32_t ny, int32_t 
  * and up/dn the vertical runs of the row, each with its widest lateral square; of those
  * containing the safe-radius cube, the one with the largest harmonic mean of rxy+1, rxy+1 and
  * (zup+zdn)/2+1 is kept, the first listed winning ties.  The engine packs it into its voxel words
- * (caps 15/31 for 16-bit words, 255/255 for 32-bit) and a flight jump stays inside it; exported as
+ * (caps 15/31 for 16-bit words, 255/255 for 32-bit) and a box flight stays inside it; exported as
  * a test hook.  No counterpart in the reference. */
 ymc_status ymc_safe_box(const uint8_t *rows, int32_t nx, int32_t ny, int32_t nz, int32_t zcap, int32_t xycap,
                         uint8_t *zup, uint8_t *zdn, uint8_t *rxy);
@@ -291,13 +292,13 @@ ymc_status ymc_engine_rng_fill(ymc_engine *e, const uint64_t *x, const uint32_t 
 
 /* Parametric-geometry test hook: the device's voxel word of every voxel of the grid g describes,
  * out[(k*ny + j)*nx + i] = row | zup << 8 | zdn << 16 | rxy << 24, the extents of the safe box a
- * flight jump may stay in (see ymc_safe_box).  No counterpart in the reference. */
+ * box flight may stay in (see ymc_safe_box).  No counterpart in the reference. */
 ymc_status ymc_engine_layered_words(ymc_engine *e, const ymc_layered *g, int32_t nx, int32_t ny, int32_t nz,
                                     uint32_t *out);
 
 /* Stream-skip test hook: advances n_streams MWC streams (gpumcml_rng.cu:38-46) past the draws of
  * `crossings` plain voxel crossings (4 draws each, crossings <= 63) with the modular skip table a
- * flight jump uses, and returns the final states; they equal those of 4*crossings draws taken one
+ * box flight uses, and returns the final states; they equal those of 4*crossings draws taken one
  * by one.  No counterpart in the reference, whose crossings draw one at a time. */
 ymc_status ymc_engine_rng_skip(ymc_engine *e, const uint64_t *x, const uint32_t *a,
                                uint32_t n_streams, uint32_t crossings, uint64_t *x_final);

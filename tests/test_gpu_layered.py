@@ -131,3 +131,46 @@ def test_layered_batch_matches_oracle(engine, multipliers):
         assert abs(d) < 3 * se, (w, d, se)
         zr = (g.radial()[w] - r.radial()[0]) / np.maximum(np.hypot(g.radial_se()[w], r.radial_se()[0]), 1e-12)
         assert np.mean(np.abs(zr) > 3) <= 2 / 32, (w, zr)
+
+
+# A weakly absorbing query (dermis mu_a 0.035/cm): packets walk 1.7e4 reference steps on average and
+# up to 5e5, against the mirror walls and through the cylinder many times.
+LONG_WALK = [42.57030400217538, 0.03539621026722863, 37.85999861225982, 18.55017331336466, 46.95608120610629,
+             54.30326081246652, 0.8404832991257235, 0.9298052533655712, 0.8632422053734459, 0.002,
+             0.1094988628650844, 0.002]
+
+
+@pytest.mark.parametrize("jump_mode", [1, 2])
+def test_long_walks_end_as_in_the_oracle(engine, multipliers, jump_mode):
+    """How long-walking packets end (escape, roulette, kill) matches the oracle on the same streams.
+    The kill rules fire on non-finite state, a negative step, the step cap and zero-length steps at
+    the reference's launch boundaries only; a build that killed 1.7% of these packets (seven times
+    the oracle's rate, mostly away from launch boundaries) shortened their walks by 12%."""
+    n = 20_000
+    P = np.array([LONG_WALK])
+    props, geo = configs.skin2_tables(P)
+    engine.set_layered(geo, configs.SKIN2_SHAPE, (configs.SKIN2_VOXEL,) * 3)
+    engine.set_optprops(*props)
+    chains = np.zeros(n, dtype=y._abi.CHAIN_DTYPE)
+    chains["count"] = 1
+    x, a = y.seed_streams(multipliers, 31, n)
+    params = configs.skin2_params(0)
+    params.pop("num_photons")
+    engine.prepare_chains(chains, x, a, flags=y.FLAG_RECORD | y.FLAG_COUNTERS, jump_mode=jump_mode, **params)
+    engine.run()
+    g = engine.read_records(n)
+    sc = configs.skin2_scene(P[0], photons=n, seed=9)
+    r = O.run(sc.rows, sc.voxel, sc.optprops(), chains, x, a, records=n,
+              **{k: v for k, v in sc.params.items() if k not in ("num_photons", "budgets")}).records
+    gf, rf = np.bincount(g["fate"], minlength=3), np.bincount(r["fate"], minlength=3)
+    print(f"jump_mode {jump_mode}: fates (escaped, frozen, killed) gpu {gf.tolist()} oracle {rf.tolist()}")
+    # killed and frozen counts within 4 Poisson standard deviations of the difference
+    for f in (1, 2):
+        assert abs(int(gf[f]) - int(rf[f])) <= 4 * np.sqrt(gf[f] + rf[f] + 1), (f, gf, rf)
+    # most kills are zero-length steps at the launch boundaries (steps 45, 90, ...)
+    dead = g["fate"] == 2
+    assert np.mean(g["steps"][dead] % 45 == 0) > 0.6
+    # escaped weight within 4 standard errors of the packet weights' difference
+    ew_g = np.where(g["fate"] == 0, g["w"], 0.0).astype(np.float64)
+    ew_r = np.where(r["fate"] == 0, r["w"], 0.0).astype(np.float64)
+    assert abs(ew_g.mean() - ew_r.mean()) < 4 * np.std(ew_g - ew_r) / np.sqrt(n) + 1e-6

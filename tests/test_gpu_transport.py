@@ -57,9 +57,9 @@ def test_draw_conversion_edges(engine, multipliers):
 
 
 def test_stream_skip_bit_exact(engine, multipliers):
-    """A flight jump skips the draws of n crossings by stepping n mod 4 crossings and taking up to two
+    """A box flight skips the draws of n crossings by stepping n mod 4 crossings and taking up to two
     modular products (Montgomery, for the multiples of 4 below 32 and for 32); the states must equal
-    those of 4n draws taken one by one, for every n a jump can take and for extreme states (the
+    those of 4n draws taken one by one, for every n a flight can take and for extreme states (the
     largest states p - 1 and p - 2, and the state 1).  States that are not MWC states of their
     multiplier are refused."""
     x, a = y.seed_streams(multipliers, 4711, 1024)
@@ -326,9 +326,9 @@ def test_run_3dmc_gpu_dropin(multipliers):
 
 @pytest.mark.parametrize("scene_fn,n", [(configs.skin_slab_sphere, 20000), (configs.skin_vessels, 5000)])
 def test_flight_jumps_keep_lockstep(engine, multipliers, scene_fn, n):
-    """A jump replaces plain crossings inside a safe box by one move and advances the stream by
-    their draws: with the same stream per packet, transport with and without jumps takes the same
-    draws, and packets differ only where a rounding difference in the jumped position flips a
+    """A box flight takes the plain crossings inside a safe box in one step and advances the stream
+    by their draws: with the same stream per packet, transport with and without box flights takes the
+    same draws, and packets differ only where a rounding difference in the flight's end flips a
     later comparison.  The bars are those of the device-against-oracle lockstep test."""
     scene = scene_fn(photons=n)
     scene.load(engine)

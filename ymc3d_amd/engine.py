@@ -338,7 +338,7 @@ class Engine:
 
     def rng_skip(self, x: np.ndarray, a: np.ndarray, crossings: int) -> np.ndarray:
         """Device stream skip past the draws of `crossings` plain crossings (4 each, <= 63), with the
-        modular table a flight jump uses; returns the final states."""
+        modular table a box flight uses; returns the final states."""
         xx = np.ascontiguousarray(x, dtype=np.uint64)
         aa = np.ascontiguousarray(a, dtype=np.uint32)
         xf = np.empty_like(xx)
