@@ -200,6 +200,14 @@ __device__ __forceinline__ const RowConst &row_at(const RowConst *rows, uint32_t
                                              __umul24(idx, (uint32_t)sizeof(RowConst)));
 }
 
+// Copies a row of constants (three float4) into a lane's LDS slot.
+__device__ __forceinline__ void copy_row(RowConst *dst, const RowConst &src) {
+  const float4 r0 = src.r0, r1 = src.r1, r2 = src.r2;
+  dst->r0 = r0;
+  dst->r1 = r1;
+  dst->r2 = r2;
+}
+
 struct KArgs {
   // geometry: one word per voxel holding its row and safe box (extents zup, zdn, rxy).  Grids with
   // fewer than 8 rows: 16-bit words, row in bits 0-2, zup in 3-6, zdn in 7-10, rxy in 11-15
@@ -439,10 +447,12 @@ template <bool COUNTERS, bool LAYERED, bool RECORD>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
                                             const RowConst *rows, uint32_t rbase, unsigned long long *tot,
                                             LaneAcc &acc, const ymc_layered *lg, uint32_t table, float4 nrow,
-                                            const uint64_t *skip) {
-  // this table's rows start at rows[rbase]
-  const float4 c0 = row_at(rows, rbase + p.t).r0;
-  const float4 c1 = row_at(rows, rbase + p.t).r1;
+                                            const uint64_t *skip, RowConst *cur) {
+  // this table's rows start at rows[rbase]; parametric runs keep the packet's row in the lane's LDS
+  // slot `cur` (their tables stay in global memory)
+  const RowConst &row = LAYERED ? *cur : row_at(rows, rbase + p.t);
+  const float4 c0 = row.r0;
+  const float4 c1 = row.r1;
   const float ni = c0.w;
   const float mu_t = c1.x;
   const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
@@ -620,7 +630,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
         if (COUNTERS) atomicAdd(&tot[kRoul], (unsigned long long)((long long)fx(Wr) - (long long)fx(W)));
         W = Wr;
       }
-      const float4 c2 = row_at(rows, rbase + p.t).r2;
+      const float4 c2 = (LAYERED ? *cur : row_at(rows, rbase + p.t)).r2;
       // HG (:282-294)
       const float xi = u32_oc(u_cos);
       float ct = c2.x * rcp(fmaf(c2.z, xi, c2.y));
@@ -691,6 +701,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.i = ni_;
     p.j = nj_;
     p.k = nk_;
+    if (LAYERED && next_t != p.t) copy_row(cur, row_at(rows, rbase + next_t));
     p.t = next_t;
     // the voxel just loaded becomes the anchor (no box when the packet is leaving through the top)
     set_anchor(p, a, ni_, nj_, nk_, nw, !air_override);
@@ -767,8 +778,9 @@ __host__ __device__ inline size_t lds_tot_offset(int tables, int types) {
   size_t off = (size_t)tables * types * sizeof(RowConst) + (size_t)tables * sizeof(float);
   return (off + 7) & ~(size_t)7;
 }
+// Parametric runs hold each lane's current row of constants in LDS (one RowConst per thread).
 __host__ __device__ inline size_t lds_bytes(int tables, int types, bool layered) {
-  return layered ? 0 : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
+  return layered ? (size_t)kBlock * sizeof(RowConst) : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
 }
 
 // Occupancy.  The voxel-grid variants without counters, records or radial tally are held to 7 waves
@@ -798,6 +810,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
   }
   const RowConst *rows = LAYERED ? a.rows : s_rows;
+  RowConst *s_cur = LAYERED ? s_rows + threadIdx.x : nullptr;  // the lane's row slot (parametric runs)
   uint64_t *s_skip = a.skip_off >= 0 ? reinterpret_cast<uint64_t *>(lds_raw + a.skip_off) + threadIdx.x : nullptr;
   unsigned long long *totals = LAYERED ? a.tot : s_tot;
 
@@ -908,6 +921,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
       }
       if (left > 0) {
         launch_packet<UNIFORM, LAYERED>(p, rx, ra, a, LAYERED ? a.w0[table] : s_w0[table], &geo);
+        if (LAYERED) copy_row(s_cur, row_at(rows, rbase + p.t));
         alive = true;
       }
     }
@@ -915,7 +929,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     if (alive) {
       unsigned long long *tt = totals + (size_t)table * kTotFields;
       const int res =
-          step_packet<COUNTERS, LAYERED, RECORD>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow, s_skip);
+          step_packet<COUNTERS, LAYERED, RECORD>(p, rx, ra, a, rows, rbase, tt, acc, &geo, table, nrow, s_skip, s_cur);
       if (res != kLive) {  // tallied in the next launch phase
         alive = false;
         ended = true;

@@ -9,7 +9,7 @@ import time
 
 import numpy as np
 
-from . import configs
+from . import _abi, configs
 
 # Samples per launch, and engines (each with its own HIP stream) driven concurrently: the last,
 # long-path packets of one launch run serially while the next launch's packets fill the device.
@@ -24,21 +24,23 @@ ENGINES = 6
 CHAIN_LANES = 16 * 458752
 
 
-def _launch(engine, P, photons, seed, params):
+def _launch(engine, P, photons, seed, params, counters: bool = False):
     props, geo = configs.skin2_tables(P)
     engine.set_layered(geo, configs.SKIN2_SHAPE, (configs.SKIN2_VOXEL,) * 3)
     engine.set_optprops(*props)
-    engine.prepare(**params, budgets=photons, seed=seed, flags=1, chain_lanes=CHAIN_LANES)
+    engine.prepare(**params, budgets=photons, seed=seed, flags=_abi.FLAG_COUNTERS if counters else 0,
+                   chain_lanes=CHAIN_LANES)
     ms = engine.run()
     t = engine.read_tally()
     return ms * 1e-3, t
 
 
 def run_mc(engine, P: np.ndarray, photons, seed: int = 1, tables_per_launch: int = TABLES_PER_LAUNCH,
-           engines: int = ENGINES, progress_every: float = 20.0, label: str = "mc"):
+           engines: int = ENGINES, progress_every: float = 20.0, label: str = "mc", counters: bool = False):
     """Runs every parameter row of P; returns dict of radial [n,32], radial_se, total [n], total_se,
-    packets, steps, summed kernel seconds and wall seconds.  `engine` runs the first share of the
-    launches; engines-1 more engines are created for the rest."""
+    packets, steps (reference steps, counted only with `counters`, which runs the slower counting
+    kernel; None otherwise), summed kernel seconds and wall seconds.  `engine` runs the first share of
+    the launches; engines-1 more engines are created for the rest."""
     import concurrent.futures as cf
 
     from .engine import Engine
@@ -67,7 +69,7 @@ def run_mc(engine, P: np.ndarray, photons, seed: int = 1, tables_per_launch: int
     def work(sl):
         eng = free.get()
         try:
-            return (sl, *_launch(eng, P[sl], photons[sl], seed + sl.start, params))
+            return (sl, *_launch(eng, P[sl], photons[sl], seed + sl.start, params, counters))
         finally:
             free.put(eng)
 
@@ -78,7 +80,8 @@ def run_mc(engine, P: np.ndarray, photons, seed: int = 1, tables_per_launch: int
             sl, ks, t = f.result()
             out["kernel_s"] += ks
             out["launches"] += 1
-            out["steps"] += int(t.totals["steps"].sum())
+            if counters:
+                out["steps"] += int(t.totals["steps"].sum())
             out["radial"][sl] = t.radial()
             out["radial_se"][sl] = t.radial_se()
             out["total"][sl] = t.total_R()
@@ -91,6 +94,8 @@ def run_mc(engine, P: np.ndarray, photons, seed: int = 1, tables_per_launch: int
     for e in pool_engines[1:]:
         e.close()
     out["wall_s"] = time.perf_counter() - t0
+    if not counters:
+        out["steps"] = None
     return out
 
 

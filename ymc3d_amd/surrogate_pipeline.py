@@ -75,7 +75,8 @@ def main(argv=None) -> int:
     P, mc = D.generate(eng, a.samples, a.photons, seed=a.seed, label="train-mc")
     Yt = S.encode_targets(mc["radial"], mc["total"])
     rep["data"] = {"samples": a.samples, "packets": mc["packets"], "wall_s": mc["wall_s"], "kernel_s": mc["kernel_s"],
-                   "packets_per_s": mc["packets"] / mc["wall_s"], "steps_per_packet": mc["steps"] / mc["packets"],
+                   "packets_per_s": mc["packets"] / mc["wall_s"],
+                   "steps_per_packet": mc["steps"] / mc["packets"] if mc["steps"] is not None else None,
                    "launches": mc["launches"]}
     print("data", json.dumps(rep["data"]), flush=True)
 
@@ -115,18 +116,23 @@ def main(argv=None) -> int:
 
 def measure_cube(ens, eng, threshold: float, n_star: float, cube=(256, 256), wavelengths: int = 64,
                  outlier_frac: float = 0.05, fallback_photons: int = 1_000_000, audit_frac: float = 0.01,
-                 fallback_sample: int = 256, audit_sample: int = 256, seed: int = 0) -> dict:
+                 fallback_sample: int = 256, audit_sample: int = 256, seed: int = 0,
+                 tables_per_launch: int = 1024) -> dict:
     """Serving a hyperspectral cube with Monte Carlo fallback, measured on the whole cube.
 
     Every query of the height x width x `wavelengths` cube (`outlier_frac` of the pixels out of range)
-    goes through the ensemble (fused passes of 4096 rows) and is triaged (S.triage).  The
-    `fallback_photons` Monte Carlo runs on a seeded sample of `fallback_sample` flagged queries and of
-    `audit_sample` audit queries; the per-query fallback and audit times of those samples, times the
-    numbers of flagged and audit queries, give the Monte Carlo time of the whole cube.  The audit
+    goes through the ensemble (fused passes of 4096 rows) and is triaged (S.triage).  Flagged queries
+    and the audit share are re-run by Monte Carlo with `fallback_photons` packets each.  Seeded samples
+    of `fallback_sample` flagged and `audit_sample` audit queries run one after the other, each in
+    launches of `tables_per_launch` queries on concurrent engines; each sample's time per query, times
+    the number of flagged or audit queries, gives the Monte Carlo time of the whole cube.  The audit
     sample gives the surrogate's median relative error against Monte Carlo (with that Monte Carlo's
-    own noise, and the error left once that noise is removed in quadrature), and its packet rate
-    gives plain Monte Carlo's spectra/s at the surrogate's accuracy (`n_star` packets per query: the
-    count at which plain Monte Carlo's median relative error equals the surrogate's)."""
+    own noise, and the error left once that noise is removed in quadrature), and its packet rate gives
+    plain Monte Carlo's spectra/s at the surrogate's accuracy (`n_star` packets per query: the count
+    at which plain Monte Carlo's median relative error equals the surrogate's).
+    The flagged queries' time is an upper bound: some out-of-range queries (g near 0.98, mu_s near
+    1800/cm, mu_a near 0.01/cm) have packets that walk 4e6 reference steps, and a launch waits for
+    them; a whole-cube run of hundreds of launches overlaps those walks with other launches."""
     from ymc3d_amd import surrogate as S, surrogate_data as D
 
     H, W = cube
@@ -140,7 +146,7 @@ def measure_cube(ens, eng, threshold: float, n_star: float, cube=(256, 256), wav
         return np.sort(rng.choice(idx, size=min(k, idx.size), replace=False)) if idx.size else idx
 
     fs, aus = sample(flagged, fallback_sample), sample(audit, audit_sample)
-    mc_kw = dict(tables_per_launch=1024, progress_every=1e9)
+    mc_kw = dict(tables_per_launch=tables_per_launch, progress_every=1e9)
     # untimed warm-up launch (first use of the parametric kernel and of the fallback engines)
     D.run_mc(eng, Q[fs[:8]] if fs.size else Q[:8], 10_000, seed=seed + 700, label="warm-up", **mc_kw)
     fb = D.run_mc(eng, Q[fs], fallback_photons, seed=seed + 500, label="fallback-sample", **mc_kw) if fs.size else None
@@ -155,19 +161,21 @@ def measure_cube(ens, eng, threshold: float, n_star: float, cube=(256, 256), wav
     t_fb = fb["wall_s"] / fs.size * flagged.sum() if fb else 0.0
     t_au = au["wall_s"] / aus.size * audit.sum() if au else 0.0
     out["fallback_s_per_query"] = fb["wall_s"] / fs.size if fb else 0.0
+    out["audit_s_per_query"] = au["wall_s"] / aus.size if au else 0.0
     out["mc_s_whole_cube"] = t_fb + t_au
     out["total_s"] = tr["surrogate_s"] + t_fb + t_au
     out["spectra_per_s_with_fallback"] = H * W / out["total_s"]
     out["surrogate_only_spectra_per_s"] = H * W / tr["surrogate_s"]
+    if au:
+        out["mc_packets_per_s"] = au["packets"] / au["wall_s"]
+        out["mc_spectra_per_s_at_matched_error"] = out["mc_packets_per_s"] / (wavelengths * n_star)
+        out["speedup_with_fallback"] = out["spectra_per_s_with_fallback"] / out["mc_spectra_per_s_at_matched_error"]
     if au:
         err, ref = rel_err(tr["mean"][aus], au["radial"], au["total"])
         ase = np.concatenate([au["radial_se"], au["total_se"][:, None]], axis=1) / np.maximum(ref, 1e-12)
         out["audit_median_rel_err"] = float(np.median(err))
         out["audit_mc_median_rel_se"] = float(np.median(ase))
         out["audit_rel_err_less_mc_noise"] = float(np.sqrt(max(np.median(err) ** 2 - np.median(ase) ** 2, 0.0)))
-        out["mc_packets_per_s"] = au["packets"] / au["wall_s"]
-        out["mc_spectra_per_s_at_matched_error"] = out["mc_packets_per_s"] / (wavelengths * n_star)
-        out["speedup_with_fallback"] = out["spectra_per_s_with_fallback"] / out["mc_spectra_per_s_at_matched_error"]
     return out
 
 
