@@ -145,11 +145,20 @@ class Ensemble:
         ens.y_mean = targets.mean(0)
         ens.y_std = targets.std(0) + 1e-6
         yz = ((targets - ens.y_mean) / ens.y_std).astype(np.float32)
-        for m in range(members):
+        def fit(m):
             net = MLP(seed=seed * 1000 + m)
             # each member sees its own bootstrap resample of the data
             idx = np.random.default_rng(seed * 1000 + m).integers(0, len(x), len(x))
-            losses = net.fit(x[idx], yz[idx], epochs, batch, lr, lr * 0.01, seed * 1000 + m)
+            return net, net.fit(x[idx], yz[idx], epochs, batch, lr, lr * 0.01, seed * 1000 + m)
+
+        # Members train concurrently, each from its own host thread on its own stream: a training step
+        # is a chain of small kernels, and one thread's launches leave the device mostly idle.  Each
+        # member's result does not depend on the others (identical parameters either way).
+        import concurrent.futures as cf
+
+        with cf.ThreadPoolExecutor(max(1, members)) as ex:
+            fitted = list(ex.map(fit, range(members)))
+        for m, (net, losses) in enumerate(fitted):
             if verbose:
                 print(f"member {m}: loss {losses[0]:.4f} -> {losses[-1]:.5f}", flush=True)
             ens.members.append(net)
