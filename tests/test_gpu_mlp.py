@@ -146,3 +146,23 @@ def test_fused_gelu_equals_exact_gelu_in_bf16():
         assert np.mean(g[big] == want[big]) >= 0.995, np.mean(g[big] == want[big])
     finally:
         net.close()
+
+
+def test_concurrent_member_training_equals_sequential():
+    """Ensemble members train from concurrent host threads, each network on its own stream: every
+    member's parameters equal those of the same member trained alone."""
+    rng = np.random.default_rng(7)
+    P = np.column_stack([rng.uniform(lo, hi, 3000) for lo, hi in zip(S.configs.PARAM_LO, S.configs.PARAM_HI)])
+    targets = rng.normal(size=(3000, S.WIDTHS[-1])).astype(np.float32)
+    ens = S.Ensemble.train(P, targets, members=3, epochs=2, batch=256, seed=5)
+    try:
+        x = S.encode_params(P)
+        yz = ((targets - ens.y_mean) / ens.y_std).astype(np.float32)
+        for m, member in enumerate(ens.members):
+            alone = S.MLP(seed=5 * 1000 + m)
+            idx = np.random.default_rng(5 * 1000 + m).integers(0, len(x), len(x))
+            alone.fit(x[idx], yz[idx], 2, 256, 2e-3, 2e-5, 5 * 1000 + m)
+            assert np.array_equal(alone.get_params(), member.get_params()), m
+            alone.close()
+    finally:
+        ens.close()
