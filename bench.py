@@ -65,11 +65,16 @@ def main() -> int:
     ap.add_argument("--atomic-rate", action="store_true",
                     help="instead of the workload, time 2^27 scattered 64-bit atomic adds into 2^23 counters (the "
                          "ceiling of the absorption grid's deposits)")
+    ap.add_argument("--fma-rate", action="store_true",
+                    help="instead of the workload, time plain FP32 FMA chains at 1..8 waves per SIMD reading 1..3 "
+                         "VGPRs (what unpacked v_fma_f32 reaches against the 157.3 TFLOP/s peak)")
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_reference(args.copy_bytes, max(args.steps, 1))
     if args.atomic_rate:
         return atomic_reference(max(args.steps, 1))
+    if args.fma_rate:
+        return fma_reference(max(args.steps, 1))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the photon budget shards across GPUs with one integer all-reduce of "
@@ -363,6 +368,30 @@ def atomic_reference(reps: int) -> int:
         print("atomic reference: counter sum mismatch", file=sys.stderr)
         return 1
     print(json.dumps({"atomic_adds": n, "counters": nvox, "ms": best, "atomics_per_s": n / (best * 1e-3)}), flush=True)
+    return 0
+
+
+def fma_reference(reps: int) -> int:
+    """Independent FP32 FMA chains, eight per lane, on every SIMD (library probe, HIP events), for
+    FMAs reading 1, 2 and 3 distinct VGPRs."""
+    import ctypes as C
+
+    from ymc3d_amd.engine import check, load_library
+
+    lib = load_library()
+    rows = []
+    for operands in (1, 2, 3):
+        for waves in (1, 2, 4, 8):
+            best_t, best_ms = 0.0, 0.0
+            for _ in range(reps):
+                tf, ms = C.c_double(), C.c_double()
+                check(lib.ymc_probe_fma_rate(0, waves, operands, 20000, C.byref(tf), C.byref(ms)),
+                      "ymc_probe_fma_rate")
+                if tf.value > best_t:
+                    best_t, best_ms = tf.value, ms.value
+            rows.append({"vgpr_operands": operands, "waves_per_simd": waves, "tflops": best_t, "ms": best_ms,
+                         "frac_of_peak": best_t / PEAK_FP32_TFLOPS})
+    print(json.dumps({"fma_rate": rows}), flush=True)
     return 0
 
 

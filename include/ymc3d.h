@@ -303,6 +303,15 @@ ymc_status ymc_engine_layered_words(ymc_engine *e, const ymc_layered *g, int32_t
 ymc_status ymc_engine_rng_skip(ymc_engine *e, const uint64_t *x, const uint32_t *a,
                                uint32_t n_streams, uint32_t crossings, uint64_t *x_final);
 
+/* FP32 FMA rate probe: times `iters` x 128 v_fma_f32 per lane (eight independent chains) at
+ * `waves_per_simd` waves on every SIMD of `device`, each FMA reading `operands` distinct VGPRs
+ * (1: v*v+v; 2: v*m+c with m in an SGPR; 3: v*a+b, all per lane), and returns TFLOP/s (2 per FMA)
+ * and the kernel time.  It measures what plain, unpacked FP32 FMA reaches against the chip's
+ * published FP32 peak, the transport kernel's roofline denominator.  No counterpart in the
+ * reference. */
+ymc_status ymc_probe_fma_rate(int32_t device, int32_t waves_per_simd, int32_t operands, int32_t iters,
+                              double *tflops, double *ms);
+
 /* One-shot drop-in for run_3dmc_gpu (src/mc3d_gpu.cu:388): same arguments as plain pointers.
  * tissue holds remapped int32 rows (as geometry_t.tissue_type); optprops arrays have
  * num_types entries with row 0 = ambient.  R and T receive raw weight sums per geometry

@@ -75,3 +75,15 @@ def test_engine_without_gpu_fails_loudly():
     else:
         e.close()
         pytest.skip("a GPU is present")
+
+
+def test_fma_probe_rejects_bad_arguments_before_touching_a_device():
+    """The FMA-rate probe checks its arguments on the host: 0 or 9 waves per SIMD, operand counts
+    outside 1..3, no iterations or no output is an argument error whether or not a GPU is present."""
+    lib = y.load_library()
+    tf, ms = C.c_double(), C.c_double()
+    for waves, iters in ((0, 10), (9, 10), (4, 0)):
+        assert lib.ymc_probe_fma_rate(0, waves, 2, iters, C.byref(tf), C.byref(ms)) == _abi.ERR_ARG
+    for operands in (0, 4):
+        assert lib.ymc_probe_fma_rate(0, 4, operands, 10, C.byref(tf), C.byref(ms)) == _abi.ERR_ARG
+    assert lib.ymc_probe_fma_rate(0, 4, 2, 10, None, None) == _abi.ERR_ARG

@@ -455,3 +455,19 @@ def test_16_and_32_bit_voxel_words_agree(engine):
         else:
             assert t0["photons"] == t1["photons"]
             assert abs(byte.total_R()[0] - word.total_R()[0]) < 0.3 * byte.total_R_se()[0]
+
+
+def test_fma_probe_stays_below_the_fp32_peak(engine):
+    """The FMA-rate probe's figure is physical for every operand form: positive, not falling from 1 to
+    8 waves per SIMD (5% slack), and below MI355X's 157.3 TFLOP/s FP32 peak (plus 5% for clock boost)."""
+    lib = y.load_library()
+    rates = []
+    for operands in (1, 2, 3):
+        for waves in (1, 8):
+            tf, ms = C.c_double(), C.c_double()
+            y.engine.check(lib.ymc_probe_fma_rate(0, waves, operands, 2000, C.byref(tf), C.byref(ms)),
+                           "ymc_probe_fma_rate")
+            assert ms.value > 0
+            rates.append(tf.value)
+        assert 0 < rates[-2] <= rates[-1] * 1.05
+        assert rates[-1] < 157.3 * 1.05

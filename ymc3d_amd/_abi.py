@@ -148,6 +148,8 @@ _SIGS = {
     "ymc_engine_layered_words": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_uint32)]),
     "ymc_engine_rng_skip": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint32), C.c_uint32, C.c_uint32,
                                       P(C.c_uint64)]),
+    "ymc_probe_fma_rate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_double),
+                                     P(C.c_double)]),
     "ymc_run_3dmc_gpu": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
                                    P(C.c_int32), C.c_int32, P(C.c_float), P(C.c_float), P(C.c_float),
                                    P(C.c_float), C.c_int32, C.c_uint64, P(C.c_uint32), C.c_uint32,

@@ -983,6 +983,33 @@ __global__ __launch_bounds__(kBlock) void rng_skip_kernel(const uint64_t *x, con
 }
 
 // RNG test hook: n_streams lanes, n_draws draws each.
+// FP32 FMA rate probe: eight independent fma chains per lane, so a SIMD always has an FMA to issue.
+// Built like the rest of the library (no packed FP32).  OPERANDS is the number of distinct VGPRs an
+// FMA reads: 1 = v*v+v, 2 = v*m+c (m wave-uniform in an SGPR, c in a VGPR), 3 = v*a+b with per-lane
+// a and b, the usual case in the transport kernel.
+constexpr int kProbeChains = 8, kProbeUnroll = 16;
+template <int OPERANDS>
+__global__ __launch_bounds__(256) void fma_probe_kernel(float *out, int iters, float m, float c) {
+  float v[kProbeChains];
+#pragma unroll
+  for (int q = 0; q < kProbeChains; ++q) v[q] = (float)(threadIdx.x + q) * 1e-3f;
+  const float a = m + (float)threadIdx.x * 1e-9f, b = c * (float)(threadIdx.x & 7);
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; ++u)
+#pragma unroll
+      for (int q = 0; q < kProbeChains; ++q) {
+        if (OPERANDS == 1) v[q] = fmaf(v[q], v[q], v[q]);
+        else if (OPERANDS == 2) v[q] = fmaf(v[q], m, c);
+        else v[q] = fmaf(v[q], a, b);
+      }
+  }
+  float sum = 0.0f;
+#pragma unroll
+  for (int q = 0; q < kProbeChains; ++q) sum += v[q];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
 __global__ void rng_fill_kernel(const uint64_t *x, const uint32_t *a, uint32_t n, uint32_t draws, int kind,
                                 float *out, uint64_t *xf) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1666,6 +1693,37 @@ ymc_status ymc_engine_rng_skip(ymc_engine *e, const uint64_t *x, const uint32_t 
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(x_final, dxf.p, (size_t)n_streams * 8, hipMemcpyDeviceToHost));
+  return YMC_OK;
+}
+
+ymc_status ymc_probe_fma_rate(int32_t device, int32_t waves_per_simd, int32_t operands, int32_t iters,
+                              double *tflops, double *ms) {
+  if (!tflops || waves_per_simd < 1 || waves_per_simd > 8 || operands < 1 || operands > 3 || iters < 1)
+    return fail(YMC_ERR_ARG, "ymc_probe_fma_rate: waves_per_simd 1..8, operands 1..3, iters >= 1");
+  auto kern = operands == 1 ? fma_probe_kernel<1> : operands == 2 ? fma_probe_kernel<2> : fma_probe_kernel<3>;
+  HIP_TRY(hipSetDevice(device));
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const uint32_t blocks = (uint32_t)cus * (uint32_t)waves_per_simd;  // 4 waves per block, one per SIMD
+  DevBuf out;
+  ymc_status st;
+  if ((st = out.ensure((size_t)blocks * 256 * 4))) return st;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, nullptr, out.as<float>(), iters, 0.999999f, 1e-7f);
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, nullptr, out.as<float>(), iters, 0.999999f, 1e-7f);
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  HIP_TRY(hipEventDestroy(e0));
+  HIP_TRY(hipEventDestroy(e1));
+  HIP_TRY(hipGetLastError());
+  const double flops = 2.0 * blocks * 256.0 * iters * kProbeUnroll * kProbeChains;
+  *tflops = flops / (t * 1e-3) / 1e12;
+  if (ms) *ms = t;
   return YMC_OK;
 }
 
