@@ -45,6 +45,10 @@ PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
 
 
 def main() -> int:
+    # The surrogate-serving measurement streams Monte Carlo launches through 8 concurrent engines;
+    # each needs its own hardware queue (HIP's default is 4).  Read when HIP starts, so set first.
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -219,9 +223,9 @@ def transport(workload, photons, args, steps, warmup, absorption=False, pool_lan
 def surrogate_serving() -> dict:
     """Surrogate serving of the 256 x 256-pixel x 64-wavelength hyperspectral cube with the shipped
     ensemble: every query through the ensemble (fused passes of 4096), 1e6-packet Monte Carlo
-    fallback for flagged queries and a 1% audit; the Monte Carlo runs on seeded samples of 1024 flagged
-    and 1024 audit queries, each in launches of 256 queries on concurrent engines, and its time for the
-    whole cube is scaled from them (surrogate_pipeline.measure_cube).  Plain Monte Carlo at matched accuracy: the audit sample's
+    fallback for flagged queries and a 1% audit; the Monte Carlo runs on seeded samples of 4096 flagged
+    and 1024 audit queries, each streamed through 8 concurrent engines (16 hardware queues) in launches
+    of 512 queries, and its time for the whole cube is scaled from them (surrogate_pipeline.measure_cube).  Plain Monte Carlo at matched accuracy: the audit sample's
     packet rate over 64 x n_star packets per spectrum."""
     import ymc3d_amd as y
     from ymc3d_amd import surrogate as S, surrogate_pipeline as SP
@@ -232,7 +236,7 @@ def surrogate_serving() -> dict:
     eng = y.Engine(0)
     try:
         r = SP.measure_cube(ens, eng, float(extra["threshold"]), float(extra["n_star"]), cube=(256, 256),
-                            fallback_sample=1024, audit_sample=1024, seed=0, tables_per_launch=256)
+                            fallback_sample=4096, audit_sample=1024, seed=0, tables_per_launch=512, engines=8)
     finally:
         ens.close()
         eng.close()
@@ -248,10 +252,11 @@ def surrogate_serving() -> dict:
     for k in ("median_rel_err", "median_rel_err_total", "test_photons", "n_star"):
         if k in extra:
             out["eval_" + k] = float(extra[k])
-    out["note"] = ("Monte Carlo time of the whole cube scaled from the per-query times of the seeded fallback and "
-                   "audit samples (1024 queries each, launches of 256 on concurrent engines); the fallback time is "
-                   "an upper bound (a launch waits for the 4e6-step walks of some out-of-range queries); eval_* from "
-                   "the held-out evaluation stored with the ensemble")
+    out["hw_queues"] = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    out["note"] = ("Monte Carlo time of the whole cube scaled from the per-query times of the seeded fallback "
+                   "(4096 queries) and audit (1024) samples, each streamed through 8 concurrent engines in launches "
+                   "of 512; the fallback time is an upper bound (the sample ends on the 4e6-step walks of some "
+                   "out-of-range queries); eval_* from the held-out evaluation stored with the ensemble")
     return out
 
 

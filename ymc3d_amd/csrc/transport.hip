@@ -982,7 +982,6 @@ __global__ __launch_bounds__(kBlock) void rng_skip_kernel(const uint64_t *x, con
   xf[s] = st;
 }
 
-// RNG test hook: n_streams lanes, n_draws draws each.
 // FP32 FMA rate probe: eight independent fma chains per lane, so a SIMD always has an FMA to issue.
 // Built like the rest of the library (no packed FP32).  OPERANDS is the number of distinct VGPRs an
 // FMA reads: 1 = v*v+v, 2 = v*m+c (m wave-uniform in an SGPR, c in a VGPR), 3 = v*a+b with per-lane
@@ -1010,6 +1009,7 @@ __global__ __launch_bounds__(256) void fma_probe_kernel(float *out, int iters, f
   out[blockIdx.x * blockDim.x + threadIdx.x] = sum;
 }
 
+// RNG test hook: n_streams lanes, n_draws draws each.
 __global__ void rng_fill_kernel(const uint64_t *x, const uint32_t *a, uint32_t n, uint32_t draws, int kind,
                                 float *out, uint64_t *xf) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;

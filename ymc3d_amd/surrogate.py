@@ -223,7 +223,9 @@ def serve(ens: Ensemble, P: np.ndarray, engine, threshold: float, fallback_photo
     A row is re-run by batched Monte Carlo (`fallback_photons` packets, surrogate_data.run_mc) when it
     lies outside the training ranges or when the largest ensemble standard deviation over its 33
     outputs exceeds `threshold` (log10 units).  A seeded `audit_frac` share of the other rows is
-    re-run too, to measure the served error; those keep the surrogate answer.  Returns `pred`
+    re-run too, to measure the served error; those keep the surrogate answer.  The Monte Carlo
+    launches stream through surrogate_data.ENGINES concurrent engines, which need as many hardware
+    queues (GPU_MAX_HW_QUEUES=16 in the environment before HIP starts; HIP's default is 4).  Returns `pred`
     (log10 targets, Monte Carlo values in flagged rows), `flagged`, `audit`, `rows` (indices
     re-run), `mc` (the run_mc result for `rows`) and `surrogate_s` (inference seconds)."""
     from . import surrogate_data

@@ -39,6 +39,10 @@ def rel_err(pred_log, radial, total):
 
 
 def main(argv=None) -> int:
+    # Monte Carlo launches stream through surrogate_data.ENGINES concurrent engines, each on its own
+    # hardware queue (HIP's default is 4; read when HIP starts, before the first engine).
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=200_000)
     ap.add_argument("--photons", type=int, default=100_000)
@@ -117,22 +121,26 @@ def main(argv=None) -> int:
 def measure_cube(ens, eng, threshold: float, n_star: float, cube=(256, 256), wavelengths: int = 64,
                  outlier_frac: float = 0.05, fallback_photons: int = 1_000_000, audit_frac: float = 0.01,
                  fallback_sample: int = 256, audit_sample: int = 256, seed: int = 0,
-                 tables_per_launch: int = 1024) -> dict:
+                 tables_per_launch: int = 1024, engines: int = 8) -> dict:
     """Serving a hyperspectral cube with Monte Carlo fallback, measured on the whole cube.
 
     Every query of the height x width x `wavelengths` cube (`outlier_frac` of the pixels out of range)
     goes through the ensemble (fused passes of 4096 rows) and is triaged (S.triage).  Flagged queries
     and the audit share are re-run by Monte Carlo with `fallback_photons` packets each.  Seeded samples
-    of `fallback_sample` flagged and `audit_sample` audit queries run one after the other, each in
-    launches of `tables_per_launch` queries on concurrent engines; each sample's time per query, times
-    the number of flagged or audit queries, gives the Monte Carlo time of the whole cube.  The audit
+    of `fallback_sample` flagged and `audit_sample` audit queries run one after the other, each
+    streamed through `engines` concurrent engines in launches of `tables_per_launch` queries; each
+    sample's time per query, times the number of flagged or audit queries, gives the Monte Carlo time
+    of the whole cube.  The audit
     sample gives the surrogate's median relative error against Monte Carlo (with that Monte Carlo's
     own noise, and the error left once that noise is removed in quadrature), and its packet rate gives
     plain Monte Carlo's spectra/s at the surrogate's accuracy (`n_star` packets per query: the count
     at which plain Monte Carlo's median relative error equals the surrogate's).
     The flagged queries' time is an upper bound: some out-of-range queries (g near 0.98, mu_s near
     1800/cm, mu_a near 0.01/cm) have packets that walk 4e6 reference steps, and a launch waits for
-    them; a whole-cube run of hundreds of launches overlaps those walks with other launches."""
+    them.  While one engine's launch finishes such walks the other engines' launches fill the device,
+    which takes as many hardware queues as engines (GPU_MAX_HW_QUEUES, read when HIP starts; HIP's
+    default is 4); the sample still ends on its own last walks, which a whole-cube run of hundreds of
+    launches overlaps with other launches."""
     from ymc3d_amd import surrogate as S, surrogate_data as D
 
     H, W = cube
@@ -146,7 +154,7 @@ def measure_cube(ens, eng, threshold: float, n_star: float, cube=(256, 256), wav
         return np.sort(rng.choice(idx, size=min(k, idx.size), replace=False)) if idx.size else idx
 
     fs, aus = sample(flagged, fallback_sample), sample(audit, audit_sample)
-    mc_kw = dict(tables_per_launch=tables_per_launch, progress_every=1e9)
+    mc_kw = dict(tables_per_launch=tables_per_launch, engines=engines, progress_every=1e9)
     # untimed warm-up launch (first use of the parametric kernel and of the fallback engines)
     D.run_mc(eng, Q[fs[:8]] if fs.size else Q[:8], 10_000, seed=seed + 700, label="warm-up", **mc_kw)
     fb = D.run_mc(eng, Q[fs], fallback_photons, seed=seed + 500, label="fallback-sample", **mc_kw) if fs.size else None
