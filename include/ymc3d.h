@@ -129,9 +129,10 @@ typedef struct ymc_params {
                                   takes the plain crossings of its flight inside the anchor voxel's
                                   safe box (faces between voxels of one tissue) together, drawing
                                   what those crossings draw */
-  int32_t jump_max_crossings;  /* most crossings one box flight may take; 0 selects 64.  At most 63
+  int32_t jump_max_crossings;  /* most crossings one box flight may take; 0 selects 63.  At most 63
                                   when their draws are skipped with the per-lane LDS table, which
-                                  the engine uses whenever it costs no occupancy */
+                                  the engine uses whenever it costs no occupancy (so the default's
+                                  results do not depend on that choice) */
   int32_t relaunch_lanes;      /* a lane whose packet ended waits until this many lanes of its wave
                                   wait (tally, next chain, launch run for all at once); 0 selects 12,
                                   1 launches at once */

@@ -201,11 +201,18 @@ __device__ __forceinline__ const RowConst &row_at(const RowConst *rows, uint32_t
 }
 
 // Copies a row of constants (three float4) into a lane's LDS slot.
-__device__ __forceinline__ void copy_row(RowConst *dst, const RowConst &src) {
-  const float4 r0 = src.r0, r1 = src.r1, r2 = src.r2;
-  dst->r0 = r0;
-  dst->r1 = r1;
-  dst->r2 = r2;
+// The lane's current row of constants in LDS (parametric runs).  The step reads the refractive
+// index, r1 and r2 of it, kept in three workgroup-wide arrays: 36 B per lane.
+struct LaneRow {
+  float4 *r1, *r2;
+  float *n;
+};
+__device__ __forceinline__ void set_lane_row(const LaneRow &d, const RowConst &src) {
+  const float n = src.r0.w;
+  const float4 r1 = src.r1, r2 = src.r2;
+  *d.n = n;
+  *d.r1 = r1;
+  *d.r2 = r2;
 }
 
 struct KArgs {
@@ -447,13 +454,19 @@ template <bool COUNTERS, bool LAYERED, bool RECORD>
 __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra, const KArgs &a,
                                             const RowConst *rows, uint32_t rbase, unsigned long long *tot,
                                             LaneAcc &acc, const ymc_layered *lg, uint32_t table, float4 nrow,
-                                            const uint64_t *skip, RowConst *cur) {
+                                            const uint64_t *skip, LaneRow cur) {
   // this table's rows start at rows[rbase]; parametric runs keep the packet's row in the lane's LDS
   // slot `cur` (their tables stay in global memory)
-  const RowConst &row = LAYERED ? *cur : row_at(rows, rbase + p.t);
-  const float4 c0 = row.r0;
-  const float4 c1 = row.r1;
-  const float ni = c0.w;
+  float ni;
+  float4 c1;
+  if (LAYERED) {
+    ni = *cur.n;
+    c1 = *cur.r1;
+  } else {
+    const RowConst &row = row_at(rows, rbase + p.t);
+    ni = row.r0.w;
+    c1 = row.r1;
+  }
   const float mu_t = c1.x;
   const bool upx = p.ux >= 0.0f, upy = p.uy >= 0.0f, upz = p.uz >= 0.0f;
 
@@ -630,7 +643,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
         if (COUNTERS) atomicAdd(&tot[kRoul], (unsigned long long)((long long)fx(Wr) - (long long)fx(W)));
         W = Wr;
       }
-      const float4 c2 = (LAYERED ? *cur : row_at(rows, rbase + p.t)).r2;
+      const float4 c2 = LAYERED ? *cur.r2 : row_at(rows, rbase + p.t).r2;
       // HG (:282-294)
       const float xi = u32_oc(u_cos);
       float ct = c2.x * rcp(fmaf(c2.z, xi, c2.y));
@@ -701,7 +714,7 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     p.i = ni_;
     p.j = nj_;
     p.k = nk_;
-    if (LAYERED && next_t != p.t) copy_row(cur, row_at(rows, rbase + next_t));
+    if (LAYERED && next_t != p.t) set_lane_row(cur, row_at(rows, rbase + next_t));
     p.t = next_t;
     // the voxel just loaded becomes the anchor (no box when the packet is leaving through the top)
     set_anchor(p, a, ni_, nj_, nk_, nw, !air_override);
@@ -778,19 +791,23 @@ __host__ __device__ inline size_t lds_tot_offset(int tables, int types) {
   size_t off = (size_t)tables * types * sizeof(RowConst) + (size_t)tables * sizeof(float);
   return (off + 7) & ~(size_t)7;
 }
-// Parametric runs hold each lane's current row of constants in LDS (one RowConst per thread).
+// Parametric runs hold each lane's current row of constants in LDS (LaneRow: 36 B per thread).
 __host__ __device__ inline size_t lds_bytes(int tables, int types, bool layered) {
-  return layered ? (size_t)kBlock * sizeof(RowConst) : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
+  return layered ? (size_t)kBlock * (2 * sizeof(float4) + sizeof(float))
+                 : lds_tot_offset(tables, types) + (size_t)tables * kTotFields * 8;
 }
 
 // Occupancy.  The voxel-grid variants without counters, records or radial tally are held to 7 waves
 // per SIMD (72 VGPRs): held to 8 (64 VGPRs) the box-flight step keeps a value in scratch, and the
-// tuned pool of the 256x256x128 layered-skin workload runs 7 waves per SIMD either way.  The other
-// variants keep the compiler's choice: the counter, record, radial and parametric variants spill to
-// scratch at 7 waves.
+// tuned pool of the 256x256x128 layered-skin workload runs 7 waves per SIMD either way.  The
+// parametric variants without counters or records are held to 6 (80 VGPRs; two or three values go
+// to scratch, one of them reloaded for a skip product), which with the skip table (25 KB of LDS per
+// workgroup) ran the surrogate's Monte Carlo 3.5% faster than the compiler's 5 waves.  The other
+// variants keep the compiler's choice: the counter, record and radial variants spill to scratch at 7.
 template <bool UNIFORM, bool COUNTERS, bool RADIAL, bool RECORD, bool LAYERED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
-    (!COUNTERS && !RECORD && !LAYERED && !RADIAL) ? 7 : 1, 8))) void transport_kernel(KArgs a) {
+    (!COUNTERS && !RECORD && !LAYERED && !RADIAL) ? 7 : ((!COUNTERS && !RECORD && LAYERED) ? 6 : 1),
+    8))) void transport_kernel(KArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   // Voxel-grid runs (few tables, many packets each) keep the optical-property rows, launch
   // weights and the workgroup's per-table totals in LDS; the totals are accumulated with LDS
@@ -810,7 +827,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
   }
   const RowConst *rows = LAYERED ? a.rows : s_rows;
-  RowConst *s_cur = LAYERED ? s_rows + threadIdx.x : nullptr;  // the lane's row slot (parametric runs)
+  LaneRow s_cur{nullptr, nullptr, nullptr};  // the lane's row slot (parametric runs)
+  if (LAYERED) {
+    float4 *r1 = reinterpret_cast<float4 *>(lds_raw);
+    s_cur = LaneRow{r1 + threadIdx.x, r1 + kBlock + threadIdx.x, reinterpret_cast<float *>(r1 + 2 * kBlock) + threadIdx.x};
+  }
   uint64_t *s_skip = a.skip_off >= 0 ? reinterpret_cast<uint64_t *>(lds_raw + a.skip_off) + threadIdx.x : nullptr;
   unsigned long long *totals = LAYERED ? a.tot : s_tot;
 
@@ -921,7 +942,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
       }
       if (left > 0) {
         launch_packet<UNIFORM, LAYERED>(p, rx, ra, a, LAYERED ? a.w0[table] : s_w0[table], &geo);
-        if (LAYERED) copy_row(s_cur, row_at(rows, rbase + p.t));
+        if (LAYERED) set_lane_row(s_cur, row_at(rows, rbase + p.t));
         alive = true;
       }
     }
@@ -1523,7 +1544,7 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   a.cancel = e->cancel_dev;
   a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
   a.jump_on = prm.jump_mode != 2;
-  a.jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : 64;
+  a.jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : kSkipMax;
   a.relaunch_lanes = prm.relaunch_lanes > 0 ? prm.relaunch_lanes : 12;
   a.relaunch_wait = prm.relaunch_wait > 0 ? prm.relaunch_wait : 16;
   a.records = e->records.as<ymc_photon_record>();
