@@ -172,6 +172,20 @@ __device__ __forceinline__ uint64_t fx_sq(uint64_t f) {
   return (f >> 32) ? low + (1ull << 32) + 2ull * fl : low;
 }
 
+// min(max(v, lo), hi) for lo <= hi as one v_med3_i32: the compiler issues a max and a min unless it
+// can prove lo <= hi.  Callers guarantee the order.
+__device__ __forceinline__ int clamp_vv(int v, int lo, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "v"(hi));
+  return r;
+}
+// The same against [0, hi] with a wave-uniform hi >= 0.
+__device__ __forceinline__ int clamp_0s(int v, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v), "s"(hi));
+  return r;
+}
+
 // photontrack_get_grid_coord (src/mc3d_gpu.cu:53-63), IEEE division; used at launch and tally.
 __device__ __forceinline__ int grid_coord(float x, float ux, float dx) {
   float q = floorf(x / dx);
@@ -262,8 +276,8 @@ struct KArgs {
   unsigned long long n_chains;
   const volatile int *cancel;
   int poll_interval;
-  int jump_on;   // box flights enabled (the plain crossings inside a safe box taken together)
-  int jump_cap;  // most plain crossings one box flight may take
+  int jump_cap;  // most plain crossings one box flight may take (at most kSkipMax with a skip table);
+                 // -1 with box flights off, so that every box flight is redone in the voxel
   int skip_off;  // byte offset of the per-lane stream-skip table in dynamic LDS, or -1 (then the
                  // skipped draws are taken one by one)
   int relaunch_lanes;  // the launch phase runs when this many lanes of a wave wait for it,
@@ -332,9 +346,9 @@ constexpr uint32_t kMaxSteps = 1u << 24;
 
 // Grids are limited to 2^31 voxels at upload, so the clamped index fits 32-bit arithmetic.
 __device__ __forceinline__ uint32_t voxel_word(const KArgs &a, int i, int j, int k) {
-  const uint32_t ic = (uint32_t)min(max(i, 0), a.nx - 1);
-  const uint32_t jc = (uint32_t)min(max(j, 0), a.ny - 1);
-  const uint32_t kc = (uint32_t)min(max(k, 0), a.nz - 1);
+  const uint32_t ic = (uint32_t)clamp_0s(i, a.nx - 1);
+  const uint32_t jc = (uint32_t)clamp_0s(j, a.ny - 1);
+  const uint32_t kc = (uint32_t)clamp_0s(k, a.nz - 1);
   const uint32_t idx = (kc * (uint32_t)a.ny + jc) * (uint32_t)a.nx + ic;
   return a.vox16 ? (uint32_t)a.vox16[idx] : a.vox32[idx];
 }
@@ -353,9 +367,9 @@ __device__ __forceinline__ int next_index(int i, float u, float n, float b) {
 // reaching further sideways, beside or above the cylinder, measured no faster on these
 // geometries, whose long walks stay deep in the bottom layer where the cube is already large.)
 __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layered &g, int i, int j, int k) {
-  const int ic = min(max(i, 0), a.nx - 1);
-  const int jc = min(max(j, 0), a.ny - 1);
-  const int kc = min(max(k, 0), a.nz - 1);
+  const int ic = clamp_0s(i, a.nx - 1);
+  const int jc = clamp_0s(j, a.ny - 1);
+  const int kc = clamp_0s(k, a.nz - 1);
   const bool top = kc < g.top_voxels;
   int row = top ? g.row_top : g.row_bottom;
   int r = top ? min(kc, g.top_voxels - 1 - kc) : kc - g.top_voxels;
@@ -421,7 +435,7 @@ __device__ __forceinline__ int axis_voxel(float c, float inv_d, int lo, int hi) 
   // floor and conversion in one instruction (the compiler issues v_floor_f32 + v_cvt_i32_f32)
   int q;
   asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(q) : "v"(c * inv_d));
-  return min(max(q, lo), hi);
+  return clamp_vv(q, lo, hi);
 }
 
 // Outcome of one step.
@@ -478,10 +492,10 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   // An x or y index outside the grid (a packet launched on the far wall) anchors a box that reaches
   // past the walls: such a packet steps in its voxel.
   const bool outside = ((uint32_t)p.i >= (uint32_t)a.nx) | ((uint32_t)p.j >= (uint32_t)a.ny);
-  bool box = a.jump_on && has_box(p) && (mu_t > 0.0f) && !on_face && !outside;
+  bool box = has_box(p) && (mu_t > 0.0f) && !on_face && !outside;
   // skip table present (a kernel argument, so the tests below are wave-uniform)
   const bool has_skip = a.skip_off >= 0;
-  const int cap = has_skip ? min(a.jump_cap, kSkipMax) : a.jump_cap;
+  const int cap = a.jump_cap;
 
   float db, dbm, nx, ny, nz, bx, by, bz;
   int ei, ej, ek, n;
@@ -1543,8 +1557,8 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   *e->cancel_host = (prm.cancel && *prm.cancel) ? 1 : 0;
   a.cancel = e->cancel_dev;
   a.poll_interval = prm.compact_interval > 0 ? prm.compact_interval : 1024;
-  a.jump_on = prm.jump_mode != 2;
-  a.jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : kSkipMax;
+  const bool jump_on = prm.jump_mode != 2;
+  const int jump_cap = prm.jump_max_crossings > 0 ? prm.jump_max_crossings : kSkipMax;
   a.relaunch_lanes = prm.relaunch_lanes > 0 ? prm.relaunch_lanes : 12;
   a.relaunch_wait = prm.relaunch_wait > 0 ? prm.relaunch_wait : 16;
   a.records = e->records.as<ymc_photon_record>();
@@ -1564,8 +1578,10 @@ ymc_status ymc_engine_run(ymc_engine *e) {
   int per_cu = 0, per_cu_skip = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds0));
   if (lds1 <= 64 * 1024) HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_skip, kern, kBlock, lds1));
-  const bool skip_table = a.jump_on && per_cu_skip >= per_cu && per_cu_skip > 0;
+  const bool skip_table = jump_on && per_cu_skip >= per_cu && per_cu_skip > 0;
   a.skip_off = skip_table ? (int)skip_off : -1;
+  // Box flights off: a cap of -1 fails every box flight's check, and the step redoes it in the voxel.
+  a.jump_cap = !jump_on ? -1 : (skip_table ? std::min(jump_cap, kSkipMax) : jump_cap);
   const size_t lds = skip_table ? lds1 : lds0;
   e->last_skip_table = skip_table;
   if (per_cu < 1) per_cu = 1;
