@@ -744,9 +744,13 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
   bool zero_kill = false;
   if (db == 0.0f)  // rare: the launch-boundary test runs only for lanes that need it
     zero_kill = a.kill0 == 2 || (a.kill0 == 1 && (p.nstep % kLaunchSteps) == 0);
+  // Non-finite state: v * 0 is +-0 for finite v and NaN for an infinity or a NaN, and a sum of +-0
+  // stays +-0, so the sum below is NaN exactly when one of the seven is not finite (multiplies and
+  // fmas with an inline zero, where seven isfinite tests were seven compares against an SGPR).
+  const float nf = fmaf(p.x, 0.0f, p.y * 0.0f) + fmaf(p.z, 0.0f, p.s * 0.0f) +
+                   fmaf(p.ux, 0.0f, fmaf(p.uy, 0.0f, p.uz * 0.0f));
   // bitwise, not short-circuit: every test is a compare, and no branch splits the reset below
-  const bool bad = !isfinite(p.x) | !isfinite(p.y) | !isfinite(p.z) | !isfinite(p.s) | !isfinite(p.ux) |
-                   !isfinite(p.uy) | !isfinite(p.uz) | zero_kill | (db < 0.0f) | (p.nstep >= kMaxSteps);
+  const bool bad = (nf != nf) | zero_kill | (db < 0.0f) | (p.nstep >= kMaxSteps);
   if (bad) {
     if (COUNTERS) {
       atomicAdd(&tot[kKilled], 1ull);
