@@ -134,3 +134,22 @@ def test_table_budget_limit(engine):
     assert ei.value.status == y._abi.ERR_ARG
     chains["count"] = [1 << 30, 1 << 30]  # exactly 2^31: accepted
     engine.prepare_chains(chains, x, a, **params)
+
+
+def test_g_zero_direction_is_killed_at_the_first_scattering(engine, multipliers):
+    """g = 0 gives 1/(2g) = inf, so the first scattering leaves a NaN direction and the packet is
+    killed (quirk Q3: src/mc3d_gpu.cu:282-283 with the non-finite kill of :343-353).  In a slab eight
+    mean free paths thick nearly every packet scatters before it leaves: the GPU kills the same
+    number of packets as the oracle on the same streams, nearly all of them, and the energy identity
+    holds exactly."""
+    n = 20000
+    s = _scene(np.zeros((8, 8, 8)), (0.01,) * 3, [[[1.0, 100.0, 0.0, 1.4]]], num_photons=n, seed=11,
+               illumination=0)
+    g, r = _both(engine, multipliers, s)
+    kg, kr = int(g.totals[0]["killed"]), int(r.totals[0]["killed"])
+    print(f"killed gpu {kg} oracle {kr} of {n}")
+    assert kg == kr
+    assert kg > 0.9 * n
+    assert int(g.totals[0]["photons"]) == n
+    _energy_exact(g, s)
+
