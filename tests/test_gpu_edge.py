@@ -11,7 +11,7 @@ import pytest
 
 import ymc3d_amd as y
 from oracle import oracle as O
-from ymc3d_amd import configs
+from ymc3d_amd import _abi, configs
 
 pytestmark = pytest.mark.gpu
 FX = 2.0 ** 32
@@ -152,4 +152,20 @@ def test_g_zero_direction_is_killed_at_the_first_scattering(engine, multipliers)
     assert kg > 0.9 * n
     assert int(g.totals[0]["photons"]) == n
     _energy_exact(g, s)
+
+
+def test_grid_dimension_limit(engine):
+    """Every grid dimension stays below 2^24 voxels, where voxel indices convert to float exactly (the
+    step computes a voxel's upper face as ((float)i + 1) * dx): a 2^24-voxel axis is refused, one
+    voxel less is taken, for voxel grids and parametric geometries alike."""
+    big = np.zeros((1, 1, 1 << 24), np.uint8)
+    with pytest.raises(y.YmcError) as e:
+        engine.set_geometry(big, 0.01, 0.01, 0.01)
+    assert "2^24" in str(e.value)
+    engine.set_geometry(big[:, :, 1:], 0.01, 0.01, 0.01)
+    geo = np.zeros(1, dtype=_abi.LAYERED_DTYPE)
+    geo["row_top"], geo["row_bottom"], geo["row_vessel"] = 1, 1, -1
+    with pytest.raises(y.YmcError):
+        engine.set_layered(geo, (1 << 24, 1, 1), (0.01, 0.01, 0.01))
+    engine.set_layered(geo, ((1 << 24) - 1, 1, 1), (0.01, 0.01, 0.01))
 

@@ -119,6 +119,7 @@ __device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t 
 // STRIDE apart (one column per lane of the workgroup).
 constexpr int kSkipEntries = 8;
 constexpr int kSkipMax = 63;  // crossings one skip covers
+constexpr int kMaxAxis = 1 << 24;  // voxels per grid dimension, below which indices convert to float exactly
 template <int STRIDE>
 __device__ __forceinline__ void build_skip(uint64_t *tab, uint32_t a) {
   uint64_t e4 = 1;
@@ -535,9 +536,11 @@ __device__ __forceinline__ int step_packet(Packet &p, uint64_t &rx, uint32_t ra,
     ei = axis_voxel(nx, a.idx, loi, hii);
     ej = axis_voxel(ny, a.idy, loj, hij);
     ek = axis_voxel(nz, a.idz, lok, hik);
-    const float lox = (float)ei * a.dx, hix = (float)(ei + 1) * a.dx;
-    const float loy = (float)ej * a.dy, hiy = (float)(ej + 1) * a.dy;
-    const float loz = (float)ek * a.dz, hiz = (float)(ek + 1) * a.dz;
+    // (float)(i + 1) is (float)i + 1 exactly: every grid dimension is below kMaxAxis = 2^24
+    const float fei = (float)ei, fej = (float)ej, fek = (float)ek;
+    const float lox = fei * a.dx, hix = (fei + 1.0f) * a.dx;
+    const float loy = fej * a.dy, hiy = (fej + 1.0f) * a.dy;
+    const float loz = fek * a.dz, hiz = (fek + 1.0f) * a.dz;
     bx = upx ? hix : lox;
     by = upy ? hiy : loy;
     bz = upz ? hiz : loz;
@@ -1242,7 +1245,7 @@ ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t
   ymc_status st = check_engine(e);
   if (st) return st;
   if (!per_table || num_tables < 1 || nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0) || !(dy > 0) || !(dz > 0) ||
-      (size_t)nx * ny * nz >= (1ull << 31))
+      (size_t)nx * ny * nz >= (1ull << 31) || std::max(nx, std::max(ny, nz)) >= kMaxAxis)
     return fail(YMC_ERR_ARG, "ymc_engine_set_layered: invalid arguments");
   for (int w = 0; w < num_tables; ++w) {
     const ymc_layered &g = per_table[w];
@@ -1289,6 +1292,8 @@ ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g) {
     return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: invalid geometry");
   size_t nv = (size_t)g->nx * g->ny * g->nz;
   if (nv >= (1ull << 31)) return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: more than 2^31 voxels");
+  if (std::max(g->nx, std::max(g->ny, g->nz)) >= kMaxAxis)
+    return fail(YMC_ERR_ARG, "ymc_engine_set_geometry: a dimension of 2^24 voxels or more");
   std::vector<uint8_t> radius(nv), zup(nv), zdn(nv), rxy(nv);
   safe_radius(g->tissue, g->nx, g->ny, g->nz, radius.data());
   uint8_t max_row = 0;
