@@ -67,7 +67,7 @@ enum { YMC_FRESNEL_YMC3D = 0, YMC_FRESNEL_EXACT = 1 };
  * (src/mc3d_io.cu:107-169) has remapped the file values: file value v in 0..254 becomes row
  * v+1, file value 255 becomes row 0 (ambient).  Rows therefore fit in uint8. */
 typedef struct ymc_geometry {
-  int32_t nx, ny, nz;     /* voxel counts */
+  int32_t nx, ny, nz;     /* voxel counts: each below 2^24, fewer than 2^31 in all */
   float dx, dy, dz;       /* voxel size, same length unit as 1/mu */
   const uint8_t *tissue;  /* nz*ny*nx rows, x fastest */
 } ymc_geometry;
@@ -254,8 +254,9 @@ void ymc_engine_destroy(ymc_engine *e);
 
 /* Uploads the voxel grid (the reference's 3-D texture, src/mc3d_gpu.cu:398-421). */
 ymc_status ymc_engine_set_geometry(ymc_engine *e, const ymc_geometry *g);
-/* Selects a parametric geometry instead of a voxel grid: grid size nx*ny*nz of voxels dx*dy*dz,
- * geometry per table (num_tables entries, matching the optical-property tables at run time). */
+/* Selects a parametric geometry instead of a voxel grid: grid size nx*ny*nz of voxels dx*dy*dz (the
+ * same limits as ymc_geometry), geometry per table (num_tables entries, matching the
+ * optical-property tables at run time). */
 ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t nz, float dx, float dy, float dz,
                                   const ymc_layered *per_table, int32_t num_tables);
 /* Uploads the optical-property tables (opticalprops_transfer_to_device, src/mc3d_io.cu:70-86). */
