@@ -1754,19 +1754,23 @@ ymc_status ymc_probe_fma_rate(int32_t device, int32_t waves_per_simd, int32_t op
   DevBuf out;
   ymc_status st;
   if ((st = out.ensure((size_t)blocks * 256 * 4))) return st;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  struct Events {  // destroyed on every return
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  HIP_TRY(hipEventCreate(&ev.e0));
+  HIP_TRY(hipEventCreate(&ev.e1));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, nullptr, out.as<float>(), iters, 0.999999f, 1e-7f);
-  HIP_TRY(hipEventRecord(e0, nullptr));
+  HIP_TRY(hipEventRecord(ev.e0, nullptr));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, nullptr, out.as<float>(), iters, 0.999999f, 1e-7f);
-  HIP_TRY(hipEventRecord(e1, nullptr));
-  HIP_TRY(hipEventSynchronize(e1));
-  float t = 0;
-  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-  HIP_TRY(hipEventDestroy(e0));
-  HIP_TRY(hipEventDestroy(e1));
+  HIP_TRY(hipEventRecord(ev.e1, nullptr));
+  HIP_TRY(hipEventSynchronize(ev.e1));
   HIP_TRY(hipGetLastError());
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, ev.e0, ev.e1));
   const double flops = 2.0 * blocks * 256.0 * iters * kProbeUnroll * kProbeChains;
   *tflops = flops / (t * 1e-3) / 1e12;
   if (ms) *ms = t;
