@@ -84,7 +84,7 @@ typedef struct ymc_layered {
   int32_t row_top, row_bottom;   /* optical-property rows of the layers */
   int32_t row_vessel;            /* row of the cylinder, or -1 for none */
   float vessel_x, vessel_z;      /* cylinder axis, voxel units */
-  float vessel_r;                /* cylinder radius, voxel units */
+  float vessel_r;                /* cylinder radius, voxel units, below 2^14 */
   float reserved;
 } ymc_layered;
 

@@ -120,6 +120,7 @@ __device__ __forceinline__ uint64_t mwc_mulmod(uint64_t v, uint64_t t, uint32_t 
 constexpr int kSkipEntries = 8;
 constexpr int kSkipMax = 63;  // crossings one skip covers
 constexpr int kMaxAxis = 1 << 24;  // voxels per grid dimension, below which indices convert to float exactly
+constexpr float kMaxVesselRadius = 16384.0f;  // voxels; bounds the rounding of the box's cylinder gap
 template <int STRIDE>
 __device__ __forceinline__ void build_skip(uint64_t *tab, uint32_t a) {
   uint64_t e4 = 1;
@@ -381,8 +382,12 @@ __device__ __forceinline__ uint32_t layered_word(const KArgs &a, const ymc_layer
     const float fz = ((float)kc + 0.5f) - g.vessel_z;
     const float d2 = fx * fx + fz * fz;
     if (d2 <= g.vessel_r * g.vessel_r) row = g.row_vessel;
-    // voxels within Chebyshev distance r have centres within r*sqrt(2) in the xz plane
-    const float gap = fabsf(sqrtf(d2) - g.vessel_r) - 0.01f;
+    // voxels within Chebyshev distance r have centres within r*sqrt(2) in the xz plane.  The root
+    // need not be correctly rounded: extents are capped at kSafeMax = 31, so the cylinder limits a
+    // box only where gap < 44, i.e. d < vessel_r + 45 < 2^15 (radii are held below 2^14 voxels),
+    // and there the roundings of d2 and the 1 ulp of v_sqrt_f32 together stay under 2^-22 d, below
+    // 0.008 voxels: inside the 0.01-voxel margin.  The rows (the test above) use no root.
+    const float gap = fabsf(__builtin_amdgcn_sqrtf(d2) - g.vessel_r) - 0.01f;
     r = min(r, gap > 0.0f ? (int)(gap * 0.70710678f) : 0);
   }
   r = min(r, min(min(ic, a.nx - 1 - ic), min(jc, a.ny - 1 - jc)));
@@ -1250,7 +1255,8 @@ ymc_status ymc_engine_set_layered(ymc_engine *e, int32_t nx, int32_t ny, int32_t
   for (int w = 0; w < num_tables; ++w) {
     const ymc_layered &g = per_table[w];
     if (g.top_voxels < 0 || g.row_top < 0 || g.row_top > 255 || g.row_bottom < 0 || g.row_bottom > 255 ||
-        g.row_vessel > 255 || !(g.vessel_r >= 0) || !std::isfinite(g.vessel_x) || !std::isfinite(g.vessel_z))
+        g.row_vessel > 255 || !(g.vessel_r >= 0 && g.vessel_r < kMaxVesselRadius) || !std::isfinite(g.vessel_x) ||
+        !std::isfinite(g.vessel_z))
       return fail(YMC_ERR_ARG, "layered geometry %d invalid", w);
   }
   if ((st = e->layered_buf.ensure(num_tables * sizeof(ymc_layered)))) return st;
@@ -1701,7 +1707,8 @@ ymc_status ymc_engine_layered_words(ymc_engine *e, const ymc_layered *g, int32_t
                                     uint32_t *out) {
   ymc_status st = check_engine(e);
   if (st) return st;
-  if (!g || !out || nx <= 0 || ny <= 0 || nz <= 0 || (size_t)nx * ny * nz >= (1ull << 31))
+  if (!g || !out || nx <= 0 || ny <= 0 || nz <= 0 || (size_t)nx * ny * nz >= (1ull << 31) ||
+      std::max(nx, std::max(ny, nz)) >= kMaxAxis || !(g->vessel_r >= 0 && g->vessel_r < kMaxVesselRadius))
     return fail(YMC_ERR_ARG, "ymc_engine_layered_words: bad arguments");
   const size_t n = (size_t)nx * ny * nz;
   DevBuf dout;
