@@ -33,7 +33,7 @@ PEAK_FP32_TFLOPS = 157.3
 # script's default workload, scaled by the same counters read on a 1 GiB device copy (x2.000 for
 # FETCH_SIZE, x1.000 for WRITE_SIZE); profiles/transport_skin_vessels_256x256x128.md.  The launch
 # is deterministic (same chains and streams every step), so the figure holds for every default run.
-MEASURED_TRAFFIC = {("skin_vessels", 100_000_000, 458752): 7.368e11}
+MEASURED_TRAFFIC = {("skin_vessels", 100_000_000, 458752): 7.385e11}
 
 WORKLOADS = {
     "skin_vessels": ("skin_vessels", 100_000_000),
