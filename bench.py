@@ -351,7 +351,8 @@ def copy_reference(nbytes: int, reps: int) -> int:
 
 def atomic_reference(reps: int) -> int:
     """Random 64-bit atomic adds: torch's index_add_ on int64 issues one device atomic per index.  A
-    64 MiB grid of counters, like the C2 absorption grid, and uniform random indices."""
+    64 MiB grid of counters, like the absorption grid of the 256x256x128 vessel workload, and uniform
+    random indices."""
     import torch
 
     nvox, n = 1 << 23, 1 << 27

@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def served(engine):
     ens, extra = S.Ensemble.load(S.DEFAULT_ENSEMBLE)
-    print("C5: 256x256x64 cube, fallback sample 128, audit sample 256", flush=True)
+    print("256x256x64 cube, fallback sample 128, audit sample 256", flush=True)
     res = SP.measure_cube(ens, engine, float(extra["threshold"]), float(extra["n_star"]), cube=(256, 256),
                           fallback_sample=128, audit_sample=256, seed=0)
     keys = ("queries", "flagged_frac", "out_of_range", "audit", "surrogate_queries_per_s", "fallback_s_per_query",

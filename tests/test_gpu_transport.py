@@ -198,7 +198,7 @@ def test_c2_sample_parity(engine, multipliers):
     rel = abs(g.total_R()[0] / r.total_R()[0] - 1)
     assert rel < 0.005, rel
     z, nb = _compare_maps(g, r, min_count=20)
-    print(f"C2 full-resolution bins compared {nb}, |z|>3 fraction {np.mean(np.abs(z) > 3):.4f}")
+    print(f"skin_vessels full-resolution bins compared {nb}, |z|>3 fraction {np.mean(np.abs(z) > 3):.4f}")
     assert nb > 20000 and np.mean(np.abs(z) > 3) <= 0.01
     # 16x16 coarse bins
     cg = g.R_w[0].reshape(16, 16, 16, 16).sum((1, 3)).astype(float)
@@ -226,11 +226,11 @@ def test_c2_parity_independent_streams(engine, multipliers):
     r = O.run_scene(scene, multipliers, seed=4099)
     d = g.total_R()[0] - r.total_R()[0]
     se = np.hypot(g.total_R_se()[0], r.total_R_se()[0])
-    print(f"C2 total R gpu {g.total_R()[0]:.6f} oracle {r.total_R()[0]:.6f} diff {d:.2e} se {se:.2e}", flush=True)
+    print(f"skin_vessels total R gpu {g.total_R()[0]:.6f} oracle {r.total_R()[0]:.6f} diff {d:.2e} se {se:.2e}", flush=True)
     assert abs(d) < 3 * se
     assert abs(d) / r.total_R()[0] < 0.005
     z, nb = _compare_maps(g, r, min_count=20)
-    print(f"C2 independent: bins {nb}, |z|>3 fraction {np.mean(np.abs(z) > 3):.4f}", flush=True)
+    print(f"skin_vessels independent: bins {nb}, |z|>3 fraction {np.mean(np.abs(z) > 3):.4f}", flush=True)
     assert nb > 20000 and np.mean(np.abs(z) > 3) <= 0.01
 
     def coarse(res):
@@ -241,7 +241,7 @@ def test_c2_parity_independent_streams(engine, multipliers):
     mg, sg = coarse(g)
     mr, sr = coarse(r)
     zc = (mg - mr) / np.hypot(sg, sr)
-    print(f"C2 independent 64x64: mean z^2 {np.mean(zc ** 2):.4f}, |z|>3 {np.mean(np.abs(zc) > 3):.4f}", flush=True)
+    print(f"skin_vessels independent 64x64: mean z^2 {np.mean(zc ** 2):.4f}, |z|>3 {np.mean(np.abs(zc) > 3):.4f}", flush=True)
     assert abs(np.mean(zc ** 2) - 1.0) < 0.1
     assert np.mean(np.abs(zc) > 3) <= 0.01
 

@@ -14,7 +14,7 @@ from . import _abi, configs
 # Samples per launch, and engines (each with its own HIP stream) driven concurrently: the last,
 # long-path packets of one launch run serially while the other launches' packets fill the device.
 # Concurrent launches need a hardware queue each: GPU_MAX_HW_QUEUES (read when HIP starts, default
-# 4) at least ENGINES.  On 4096 flagged queries of the C5 cube (1e6 packets each) 8 engines on 16
+# 4) at least ENGINES.  On 4096 flagged queries of the 256x256x64 serving cube (1e6 packets each) 8 engines on 16
 # queues took 47 s, 6 engines on 4 queues 53-56 s.
 TABLES_PER_LAUNCH = 2048
 ENGINES = 8
