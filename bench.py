@@ -5,7 +5,7 @@ Workload (default): layered skin with eight vessels, 256x256x128 voxels of 0.002
 illumination, 1e8 packets per step, 256x256 reflectance grid (configs.skin_vessels).  A step is
 one full run of the budget: tally reset, transport kernel, tally read-back to the host.  Inputs
 (voxel grid, optical-property tables, stream seeds and chain schedule) are resident on the device
-before timing starts; seeding is excluded as in the baseline definition.
+before timing starts; stream seeding is host work done once per workload and is not timed.
 
 Prints one JSON line with `roofline` (FP32 VALU work of the transport kernel over the MI355X FP32
 peak, kernel time from HIP events on the engine's stream) and `cpu_baseline` (the CPU oracle on a
