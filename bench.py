@@ -40,6 +40,7 @@ WORKLOADS = {
     "skin_slab_sphere": ("skin_slab_sphere", 1_000_000),
     "batched_spectra": ("batched_spectra", 0),
     "surrogate": ("surrogate", 0),
+    "skin2_batch": ("skin2_batch", 0),
 }
 PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
 
@@ -87,6 +88,8 @@ def main() -> int:
 
     if args.workload == "surrogate":
         return surrogate_bench(args)
+    if args.workload == "skin2_batch":
+        return parametric_bench(args)
     r = transport(args.workload, args.photons, args, args.steps, args.warmup, absorption=args.absorption,
                   pool_lanes=args.pool_lanes, chain_lanes=args.chain_lanes)
     out = {
@@ -218,6 +221,73 @@ def transport(workload, photons, args, steps, warmup, absorption=False, pool_lan
             "traffic": None if absorption else MEASURED_TRAFFIC.get((workload, n_photons, pool)),
         },
     }
+
+
+def parametric_bench(args) -> int:
+    """The surrogate's Monte Carlo: the parametric-geometry kernel (two layers and a vessel computed per
+    voxel on the device, pencil beam, 32 radial bins) on 512 in-range parameter rows of the
+    128x128x64 skin model, one table each, 2e5 packets per table (1.02e8 per step) in one persistent
+    launch.  A counter launch gives steps and Fresnel events for the roofline numerator; `warmup`
+    untimed and `steps` timed launches follow, inputs resident."""
+    import numpy as np
+
+    import ymc3d_amd as y
+    from ymc3d_amd import configs, surrogate_data as D
+
+    tables, per_table = 512, 200_000
+    P = configs.sample_skin2_params(tables, seed=7)
+    props, geo = configs.skin2_tables(P)
+    params = configs.skin2_params(0)
+    params.pop("num_photons")
+    params.update(budgets=np.full(tables, per_table, np.uint64), seed=11, chain_lanes=D.CHAIN_LANES)
+    eng = y.Engine(0, y.default_multipliers())
+    eng.set_layered(geo, configs.SKIN2_SHAPE, (configs.SKIN2_VOXEL,) * 3)
+    eng.set_optprops(*props)
+    eng.prepare(**params, flags=y.FLAG_COUNTERS)
+    eng.run()
+    counted = eng.read_tally()
+    n_steps, fresnel = int(counted.totals["steps"].sum()), int(counted.totals["fresnel"].sum())
+    n_photons = tables * per_table
+    eng.prepare(**params)
+    for _ in range(args.warmup):
+        eng.run()
+        eng.read_tally()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.run()
+        res = eng.read_tally()
+        kernel_ms.append(res.kernel_ms)
+    elapsed = time.perf_counter() - t0
+    assert int(res.totals["photons"].sum()) == n_photons
+    assert np.array_equal(res.Rr_w, counted.Rr_w)  # same chains and streams: the counter pass's tallies
+    kms = float(np.mean(kernel_ms))
+    achieved = (n_steps * FLOPS_PER_STEP + fresnel * FLOPS_PER_FRESNEL) / (kms * 1e-3) / 1e12
+    out = {
+        "metric": "photon packets/s (1 GPU)",
+        "value": args.steps * n_photons / elapsed,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"skin2_batch_{tables}x{per_table}", "tables": tables,
+                   "photons_per_step": n_photons, "voxels": list(configs.SKIN2_SHAPE),
+                   "voxel_cm": configs.SKIN2_VOXEL, "pool_lanes": eng.pool_lanes(),
+                   "steps_per_photon": n_steps / n_photons, "fresnel_per_photon": fresnel / n_photons,
+                   "kernel_ms": kms},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    eng.close()
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def surrogate_serving() -> dict:
